@@ -1,0 +1,149 @@
+/*
+ * porturb.h -- C ABI of the MI355X-native portUrb dycore step and LES closure (libporturb.so).
+ *
+ * The reference has no C ABI: its dycore and LES closure are C++ structs called from drivers
+ * through core::Coupler::run_module (model/core/coupler.h:344-380).  This header is the thin,
+ * stream-ordered C layer under the drop-in C++ API (include/porturb/...), which keeps the
+ * reference's class and method names.  Every entry point below names the reference interface it
+ * replaces.  Plain pointers and sizes only; all field pointers are DEVICE pointers to
+ * [nz][ny][nx] arrays of double (x fastest), as core::DataManager allocates them
+ * (model/core/DataManager.h:54-56,127-209).
+ *
+ * Errors: functions return PU_OK (0) or a nonzero PU_ERR_*; pu_last_error() gives the message
+ * (thread-local).  The reference aborts instead (endrun/Kokkos::abort, model/main_header.h:101-103);
+ * the C++ layer maps nonzero returns to that behaviour.
+ * Threading: one context per stream; calls on one context are not re-entrant.
+ */
+#ifndef PORTURB_H
+#define PORTURB_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PU_MAX_TRACERS 16
+
+enum { PU_OK = 0, PU_ERR_ARG = 1, PU_ERR_HIP = 2, PU_ERR_STATE = 3, PU_ERR_UNSUPPORTED = 4 };
+
+/* Boundary conditions: options bc_x / bc_y ("periodic" | "precursor") and bc_z
+ * ("solid_wall" | "periodic"), read at dynamics_rk.h:746,752-797 and les_closure.h:431-534. */
+enum { PU_BC_PERIODIC = 0, PU_BC_PRECURSOR = 1, PU_BC_SOLID_WALL = 2 };
+
+/* Arithmetic of the dycore step.  PU_FP64: everything in double (the reference's float sites
+ * promoted).  PU_FP32: FP32 state, reconstruction, fluxes and tendencies; FP64 hydrostatic
+ * background, pressure perturbation p - p_hy, global reductions; compensated (two-float) storage of
+ * rho and rho*theta across RK stages. */
+enum { PU_FP64 = 0, PU_FP32 = 1 };
+
+/* Model configuration: grid (Coupler::distribute_mpi_and_allocate_coupled_state / set_grid,
+ * coupler.h:132-229,329-341), tracer attributes (Coupler::add_tracer, coupler.h:417-447) and the
+ * Options keys the hot path reads (dynamics_rk.h:515-534, les_closure.h:36-51, sc_init.h:31-51). */
+typedef struct {
+  int    nx, ny, nz;
+  double xlen, ylen, zlen;
+  int    ord;                 /* reconstruction order: 3, 5, 7, 9 or 11 (PORTURB_ORD) */
+  int    num_tracers;
+  int    idWV;                /* option idWV: index of water_vapor (required) */
+  int    idTKE;               /* index of the TKE tracer (LES closure), -1 if none */
+  int    bc_x, bc_y, bc_z;
+  int    weno_all;            /* option weno_all (default 1) */
+  int    enable_gravity;      /* option enable_gravity (default 1) */
+  double latitude;            /* option latitude (default 0) */
+  double cfl;                 /* option cfl (default 0.6) */
+  double immersed_power;      /* option immersed_power (default 5) */
+  double R_d, R_v, cp_d, gamma_d, C0, grav, p0;
+  unsigned char tracer_positive[PU_MAX_TRACERS];
+  unsigned char tracer_adds_mass[PU_MAX_TRACERS];
+  unsigned char tracer_diffuse[PU_MAX_TRACERS];
+  double kinematic_viscosity; /* LES option kinematic_viscosity (default 0) */
+  int    dns;                 /* LES option dns (default 0) */
+  double les_total_mult, les_dissipation_mult, les_shear_prod_mult;
+  int    precision;           /* PU_FP64 | PU_FP32 */
+} pu_config;
+
+/* The coupler state one module call reads and writes: DataManager entries density_dry, uvel,
+ * vvel, wvel, temp, one entry per tracer (mass), and immersed_proportion
+ * (dynamics_rk.h:1251-1352, les_closure.h:328-414). */
+typedef struct {
+  double       *density_dry, *uvel, *vvel, *wvel, *temp;
+  double       *tracers[PU_MAX_TRACERS];
+  const double *immersed_proportion;
+} pu_coupler_state;
+
+typedef struct pu_ctx pu_ctx;
+
+/* Context: owns the device arena (dycore stage buffers, hydrostatics, immersed masks).
+ * `stream` is a hipStream_t (NULL = the default stream).  Replaces the module objects
+ * Dynamics_Euler_Stratified_WenoFV / LES_Closure and their DataManager-held helper entries. */
+int  pu_create(const pu_config *cfg, void *stream, pu_ctx **out);
+int  pu_destroy(pu_ctx *ctx);
+const char *pu_last_error(void);
+
+/* Dynamics_Euler_Stratified_WenoFV::init (dynamics_rk.h:917-1246): hydrostatic column means in
+ * FP64, isentropic halo extrapolation, hydrostatic edges, immersed halos, any_immersed10, tau. */
+int  pu_dycore_init(pu_ctx *ctx, const pu_coupler_state *cs);
+
+/* Dynamics_Euler_Stratified_WenoFV::compute_time_step (dynamics_rk.h:57-64). */
+double pu_dycore_compute_time_step(const pu_ctx *ctx);
+
+/* Dynamics_Euler_Stratified_WenoFV::time_step (dynamics_rk.h:103-137): convert, ncycles SSPRK3
+ * (time_step_rk_3_3 :279-367, enforce_immersed_boundaries :370-473, compute_tendencies :498-732),
+ * convert back.  In place on the coupler state; stream-ordered (returns before completion). */
+int  pu_dycore_time_step(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys);
+
+/* One compute_tendencies call (dynamics_rk.h:498-732) on the dynamics state converted from `cs`.
+ * tend: device [5+num_tracers][nz][ny][nx] double.  For parity checking of a single stage. */
+int  pu_dycore_tendencies(pu_ctx *ctx, const pu_coupler_state *cs, double *tend);
+
+/* Hydrostatic profiles as the dycore holds them (host outputs): hy_{dens,theta,pressure}_cells
+ * [nz+2hs] and hy_{dens,theta,pressure}_edges [nz+1] (dynamics_rk.h:961-1008,1145-1186). */
+int  pu_dycore_hydrostatics(pu_ctx *ctx, double *dens_c, double *theta_c, double *pres_c,
+                            double *dens_e, double *theta_e, double *pres_e);
+
+/* dycore_any_immersed10 and dycore_immersed_tau (host outputs, [nz][ny][nx]) (:389-425,:1127-1143). */
+int  pu_dycore_immersed_masks(pu_ctx *ctx, unsigned char *any10, int *tau);
+
+/* LES_Closure::apply (les_closure.h:30-323).  In place on the coupler state; requires idTKE. */
+int  pu_les_closure_apply(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys);
+
+/* FP64 reductions on the device: total mass sum(rho)*dV (rho = dry + mass-adding tracers), max
+ * wind speed and max |w| (the quantities of Coupler::inform_user, coupler.h:507-530). */
+int  pu_diagnostics(pu_ctx *ctx, const pu_coupler_state *cs, double *mass, double *max_wind,
+                    double *max_abs_w);
+
+/* Device memory for DataManager entries (yakl::alloc_device / free_device, DataManager.h:54-56)
+ * and host<->device copies (YAKL deep_copy_to).  Copies are synchronous. */
+int  pu_alloc_device(size_t bytes, void **ptr);
+int  pu_free_device(void *ptr);
+int  pu_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int  pu_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int  pu_memset_device(void *ptr, int value, size_t bytes);
+int  pu_synchronize(pu_ctx *ctx);            /* Kokkos::fence() on the context's stream */
+
+/* Seeded synthetic initial state (the role of custom_modules::sc_init + sc_perturb for the
+ * benchmark cases, sc_init.h:14-768, sc_perturb.h:13-136), generated on the device.
+ * case_id: PU_CASE_BUBBLE (dry warm bubble), PU_CASE_ABL_URBAN (neutral ABL over an 8x8 array of
+ * cuboid buildings), PU_CASE_WINDFARM (ABL with capping inversion).  immersed_out receives
+ * immersed_proportion. */
+enum { PU_CASE_BUBBLE = 1, PU_CASE_ABL_URBAN = 2, PU_CASE_WINDFARM = 3 };
+int  pu_case_init(const pu_config *cfg, int case_id, unsigned long long seed,
+                  pu_coupler_state *cs, double *immersed_out);
+
+/* Timing on the context's stream with HIP events.  pu_timer_* brackets a region;
+ * pu_kernel_timing turns on per-launch events around the named kernel family ("weno_x",
+ * "weno_y", "weno_z", "prep", "combine", "les") and pu_kernel_time reads the accumulated
+ * milliseconds and launch count. */
+int  pu_timer_start(pu_ctx *ctx);
+int  pu_timer_stop(pu_ctx *ctx, double *ms);
+int  pu_kernel_timing(pu_ctx *ctx, int enable);
+int  pu_kernel_time(pu_ctx *ctx, const char *name, double *total_ms, long long *count);
+
+/* Library build information (gfx target, precision variants compiled). */
+const char *pu_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PORTURB_H */

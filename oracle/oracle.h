@@ -1,0 +1,98 @@
+/*
+ * oracle.h -- CPU oracle for the portUrb WENO finite-volume dycore step and the TKE LES closure.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C FP64 restatement of the reference algorithm, used
+ * as the parity checker by tests/, by __graft_entry__.smoke() and as bench.py's CPU baseline.
+ * Product code never includes, links or calls it.
+ *
+ * It follows the reference's unfused kernel sequence:
+ *   model/modules/dynamics_rk.h   (Dynamics_Euler_Stratified_WenoFV, :25-1352)
+ *   model/modules/helpers/WenoLimiter.h (:10-96 helpers, :116-537 WenoLimiter<real,3..11>)
+ *   model/modules/helpers/TransformMatrices.h (coefficient definitions; derived here numerically)
+ *   model/modules/les_closure.h   (LES_Closure, :9-536)
+ *   model/core/coupler.h          (halo exchange semantics, :726-1165)
+ *   experiments/examples/custom_modules/sc_init.h:728-767 (immersed_proportion_halos)
+ * with every float site of the reference promoted to double (the "FP64 oracle").
+ *
+ * Array conventions: every 3-D field is [nz][ny][nx], x fastest (the reference's styleC layout).
+ * The coupler state is one contiguous block [5+num_tracers][nz][ny][nx] holding
+ * density_dry, uvel, vvel, wvel, temp, then the tracer masses in tracer order.
+ */
+#ifndef PORTURB_ORACLE_H
+#define PORTURB_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OC_MAX_TRACERS 16
+
+enum { OC_BC_PERIODIC = 0, OC_BC_PRECURSOR = 1, OC_BC_SOLID_WALL = 2 };
+
+typedef struct {
+  int    nx, ny, nz;
+  double xlen, ylen, zlen;
+  int    ord;                 /* 3, 5, 7, 9, 11 */
+  int    num_tracers;
+  int    idWV;                /* index of water_vapor among tracers (required) */
+  int    idTKE;               /* index of TKE among tracers, -1 if none (closure only) */
+  int    bc_x, bc_y, bc_z;
+  int    weno_all;            /* option weno_all (default 1) */
+  int    enable_gravity;      /* option enable_gravity (default 1) */
+  double latitude;            /* option latitude (default 0) */
+  double cfl;                 /* option cfl (default 0.6) */
+  double immersed_power;      /* option immersed_power (default 5) */
+  double R_d, R_v, cp_d, gamma_d, C0, grav, p0;
+  unsigned char tracer_positive[OC_MAX_TRACERS];
+  unsigned char tracer_adds_mass[OC_MAX_TRACERS];
+  unsigned char tracer_diffuse[OC_MAX_TRACERS];
+  double kinematic_viscosity; /* LES option kinematic_viscosity (default 0) */
+  int    dns;                 /* LES option dns (default 0) */
+  double les_total_mult, les_dissipation_mult, les_shear_prod_mult;
+  int    nthreads;            /* OpenMP threads, <=0 means the runtime default */
+} oc_config;
+
+typedef struct oc_dycore oc_dycore;
+
+/* Coefficient tables derived numerically (long double Gauss-Jordan) from their definitions.
+ * C: [S][N][N] sub-stencil cell-average -> polynomial coefficient maps; faceL/faceR: [S][ord];
+ * idealL/idealR: [S]; linL/linR: [ord]; tv: [N][N] upper-triangular quadratic form. */
+int  oc_weno_tables(int ord, double *C, double *faceL, double *faceR, double *idealL,
+                    double *idealR, double *linL, double *linR, double *tv);
+
+/* WenoLimiter<real,ord>::compute_limited_edges (WenoLimiter.h) on one ord-point stencil. */
+int  oc_weno_limited_edges(int ord, const double *stencil, int do_map, int imm_L, int imm_R,
+                           double *qL, double *qR);
+
+/* Dynamics_Euler_Stratified_WenoFV::compute_time_step (dynamics_rk.h:57-64). */
+double oc_dycore_compute_time_step(const oc_config *cfg);
+
+/* Dynamics_Euler_Stratified_WenoFV::init (dynamics_rk.h:917-1246): hydrostatic columns,
+ * immersed halos, any_immersed10, hydrostatic edges. */
+oc_dycore *oc_dycore_create(const oc_config *cfg, const double *coupler, const double *immersed_prop);
+void       oc_dycore_destroy(oc_dycore *d);
+
+/* Dynamics_Euler_Stratified_WenoFV::time_step (dynamics_rk.h:103-137), in place on the coupler. */
+int  oc_dycore_time_step(oc_dycore *d, double *coupler, double dt_phys);
+
+/* One call of compute_tendencies (dynamics_rk.h:498-732) on the dynamics state of a coupler
+ * state (after convert_coupler_to_dynamics).  tend: [5+num_tracers][nz][ny][nx]. */
+int  oc_dycore_tendencies(oc_dycore *d, const double *coupler, double *tend);
+
+/* Hydrostatic profiles: cells [nz+2hs] x3, edges [nz+1] x3. */
+int  oc_dycore_hydrostatics(const oc_dycore *d, double *dens_c, double *theta_c, double *pres_c,
+                            double *dens_e, double *theta_e, double *pres_e);
+
+/* dycore_any_immersed10 [nz][ny][nx] and dycore_immersed_tau [nz][ny][nx] (first relax call). */
+int  oc_dycore_immersed_masks(oc_dycore *d, unsigned char *any10, int *tau);
+
+/* LES_Closure::apply (les_closure.h:30-323), in place on the coupler state. */
+int  oc_les_apply(const oc_config *cfg, double *coupler, const double *immersed_prop, double dt_phys);
+
+/* Sum over cells of the dycore's full density rho (dry + mass-adding tracers), times cell volume. */
+double oc_total_mass(const oc_config *cfg, const double *coupler);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

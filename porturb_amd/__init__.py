@@ -1,0 +1,308 @@
+"""porturb_amd -- MI355X-native portUrb WENO-FV dycore step and TKE LES closure.
+
+Python access to libporturb.so through its C ABI (include/porturb.h).  The library is the
+product; this module only loads it, mirrors its structs with ctypes and offers small helpers
+(device buffers, case configs) used by the benchmark driver and the tests.  There is no CPU
+fallback: every call goes to the HIP library, and loading fails loudly when it is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libporturb.so")
+
+PU_MAX_TRACERS = 16
+PU_OK, PU_ERR_ARG, PU_ERR_HIP, PU_ERR_STATE, PU_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+PU_BC_PERIODIC, PU_BC_PRECURSOR, PU_BC_SOLID_WALL = 0, 1, 2
+PU_FP64, PU_FP32 = 0, 1
+PU_CASE_BUBBLE, PU_CASE_ABL_URBAN, PU_CASE_WINDFARM = 1, 2, 3
+
+
+class PuConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("xlen", C.c_double), ("ylen", C.c_double), ("zlen", C.c_double),
+        ("ord", C.c_int), ("num_tracers", C.c_int), ("idWV", C.c_int), ("idTKE", C.c_int),
+        ("bc_x", C.c_int), ("bc_y", C.c_int), ("bc_z", C.c_int),
+        ("weno_all", C.c_int), ("enable_gravity", C.c_int),
+        ("latitude", C.c_double), ("cfl", C.c_double), ("immersed_power", C.c_double),
+        ("R_d", C.c_double), ("R_v", C.c_double), ("cp_d", C.c_double), ("gamma_d", C.c_double),
+        ("C0", C.c_double), ("grav", C.c_double), ("p0", C.c_double),
+        ("tracer_positive", C.c_ubyte * PU_MAX_TRACERS),
+        ("tracer_adds_mass", C.c_ubyte * PU_MAX_TRACERS),
+        ("tracer_diffuse", C.c_ubyte * PU_MAX_TRACERS),
+        ("kinematic_viscosity", C.c_double), ("dns", C.c_int),
+        ("les_total_mult", C.c_double), ("les_dissipation_mult", C.c_double),
+        ("les_shear_prod_mult", C.c_double),
+        ("precision", C.c_int),
+    ]
+
+
+class PuCouplerState(C.Structure):
+    _fields_ = [
+        ("density_dry", C.c_void_p), ("uvel", C.c_void_p), ("vvel", C.c_void_p),
+        ("wvel", C.c_void_p), ("temp", C.c_void_p),
+        ("tracers", C.c_void_p * PU_MAX_TRACERS),
+        ("immersed_proportion", C.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libporturb.so (built in-tree by porturb_amd.build).  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libporturb.so is not built: run `python -m porturb_amd.build` "
+                           "(or __graft_entry__.build()) first; there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    P, I, D, S = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+    cfgp, csp = C.POINTER(PuConfig), C.POINTER(PuCouplerState)
+    sig = {
+        "pu_create": (I, [cfgp, P, C.POINTER(P)]),
+        "pu_destroy": (I, [P]),
+        "pu_last_error": (C.c_char_p, []),
+        "pu_build_info": (C.c_char_p, []),
+        "pu_dycore_init": (I, [P, csp]),
+        "pu_dycore_compute_time_step": (D, [P]),
+        "pu_dycore_time_step": (I, [P, csp, D]),
+        "pu_dycore_tendencies": (I, [P, csp, P]),
+        "pu_dycore_hydrostatics": (I, [P] + [P] * 6),
+        "pu_dycore_immersed_masks": (I, [P, P, P]),
+        "pu_les_closure_apply": (I, [P, csp, D]),
+        "pu_diagnostics": (I, [P, csp, P, P, P]),
+        "pu_alloc_device": (I, [S, C.POINTER(P)]),
+        "pu_free_device": (I, [P]),
+        "pu_memcpy_h2d": (I, [P, P, S]),
+        "pu_memcpy_d2h": (I, [P, P, S]),
+        "pu_memset_device": (I, [P, I, S]),
+        "pu_synchronize": (I, [P]),
+        "pu_case_init": (I, [cfgp, I, C.c_ulonglong, csp, P]),
+        "pu_timer_start": (I, [P]),
+        "pu_timer_stop": (I, [P, C.POINTER(D)]),
+        "pu_kernel_timing": (I, [P, I]),
+        "pu_kernel_time": (I, [P, C.c_char_p, C.POINTER(D), C.POINTER(C.c_longlong)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != PU_OK:
+        raise RuntimeError("libporturb error %d: %s" % (rc, lib().pu_last_error().decode()))
+
+
+# ------------------------------------------------------------------------------------------
+# configuration
+# ------------------------------------------------------------------------------------------
+def physical_constants(R_d=287., cp_d=1003., R_v=461., p0=1.e5, grav=9.81):
+    """The constants custom_modules::sc_init sets (sc_init.h:31-51)."""
+    cv_d = cp_d - R_d
+    gamma = cp_d / cv_d
+    kappa = R_d / cp_d
+    C0 = (R_d * p0 ** (-kappa)) ** gamma
+    return dict(R_d=R_d, cp_d=cp_d, R_v=R_v, p0=p0, grav=grav, gamma_d=gamma, C0=C0)
+
+
+def make_config(nx, ny, nz, xlen, ylen, zlen, ord=5, tracers=None, bc_x=PU_BC_PERIODIC,
+                bc_y=PU_BC_PERIODIC, bc_z=PU_BC_SOLID_WALL, precision=PU_FP64, **opts):
+    """Config dict.  tracers: list of (name, positive, adds_mass, diffuse) in tracer order;
+    water_vapor is required by the dycore (option idWV), TKE by the LES closure."""
+    if tracers is None:
+        tracers = [("water_vapor", True, True, True)]
+    names = [t[0] for t in tracers]
+    cfg = dict(nx=nx, ny=ny, nz=nz, xlen=xlen, ylen=ylen, zlen=zlen, ord=ord,
+               tracers=list(tracers), num_tracers=len(tracers),
+               idWV=names.index("water_vapor"),
+               idTKE=names.index("TKE") if "TKE" in names else -1,
+               bc_x=bc_x, bc_y=bc_y, bc_z=bc_z, weno_all=1, enable_gravity=1, latitude=0.,
+               cfl=0.6, immersed_power=5., kinematic_viscosity=0., dns=0, les_total_mult=1.,
+               les_dissipation_mult=1., les_shear_prod_mult=1., precision=precision)
+    cfg.update(physical_constants())
+    cfg.update(opts)
+    return cfg
+
+
+def to_pu_config(cfg):
+    c = PuConfig()
+    for name, _ in PuConfig._fields_:
+        if name in ("tracer_positive", "tracer_adds_mass", "tracer_diffuse"):
+            continue
+        setattr(c, name, cfg[name])
+    for i, (_, pos, mass, diff) in enumerate(cfg["tracers"]):
+        c.tracer_positive[i] = int(pos)
+        c.tracer_adds_mass[i] = int(mass)
+        c.tracer_diffuse[i] = int(diff)
+    return c
+
+
+# ------------------------------------------------------------------------------------------
+# device buffers and the coupler state
+# ------------------------------------------------------------------------------------------
+class DeviceBuffer:
+    """A device allocation made through the library (pu_alloc_device)."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        check(lib().pu_alloc_device(nbytes, C.byref(p)))
+        self.ptr, self.nbytes = p.value, nbytes
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes == self.nbytes
+        check(lib().pu_memcpy_h2d(self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def download(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes == self.nbytes
+        check(lib().pu_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().pu_free_device(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CouplerState:
+    """Device-resident coupler entries density_dry, uvel, vvel, wvel, temp, tracers,
+    immersed_proportion ([nz][ny][nx] float64 each), as core::DataManager holds them."""
+
+    FIELDS = ("density_dry", "uvel", "vvel", "wvel", "temp")
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.shape = (cfg["nz"], cfg["ny"], cfg["nx"])
+        n = int(np.prod(self.shape)) * 8
+        self.bufs = {f: DeviceBuffer(n) for f in self.FIELDS}
+        self.tracers = [DeviceBuffer(n) for _ in range(cfg["num_tracers"])]
+        self.immersed = DeviceBuffer(n)
+        self.c = PuCouplerState()
+        for f in self.FIELDS:
+            setattr(self.c, f, self.bufs[f].ptr)
+        for i, b in enumerate(self.tracers):
+            self.c.tracers[i] = b.ptr
+        self.c.immersed_proportion = self.immersed.ptr
+
+    def upload(self, host):
+        """host: array [5+ntr][nz][ny][nx] (coupler order) and host['imm'] optional."""
+        state, imm = host
+        for i, f in enumerate(self.FIELDS):
+            self.bufs[f].upload(state[i])
+        for t, b in enumerate(self.tracers):
+            b.upload(state[5 + t])
+        if imm is not None:
+            self.immersed.upload(imm)
+
+    def download(self):
+        nv = 5 + len(self.tracers)
+        out = np.empty((nv,) + self.shape)
+        for i, f in enumerate(self.FIELDS):
+            out[i] = self.bufs[f].download(np.float64, self.shape)
+        for t, b in enumerate(self.tracers):
+            out[5 + t] = b.download(np.float64, self.shape)
+        return out
+
+    def download_immersed(self):
+        return self.immersed.download(np.float64, self.shape)
+
+
+class Dycore:
+    """A libporturb context: Dynamics_Euler_Stratified_WenoFV + LES_Closure for one config."""
+
+    def __init__(self, cfg, stream=None):
+        self.cfg = cfg
+        self.pc = to_pu_config(cfg)
+        p = C.c_void_p()
+        check(lib().pu_create(C.byref(self.pc), stream, C.byref(p)))
+        self.ctx = p.value
+
+    def init(self, cs):
+        check(lib().pu_dycore_init(self.ctx, C.byref(cs.c)))
+
+    def compute_time_step(self):
+        return lib().pu_dycore_compute_time_step(self.ctx)
+
+    def time_step(self, cs, dt):
+        check(lib().pu_dycore_time_step(self.ctx, C.byref(cs.c), dt))
+
+    def les_apply(self, cs, dt):
+        check(lib().pu_les_closure_apply(self.ctx, C.byref(cs.c), dt))
+
+    def synchronize(self):
+        check(lib().pu_synchronize(self.ctx))
+
+    def tendencies(self, cs):
+        nv = 5 + self.cfg["num_tracers"]
+        shape = (nv, self.cfg["nz"], self.cfg["ny"], self.cfg["nx"])
+        buf = DeviceBuffer(int(np.prod(shape)) * 8)
+        check(lib().pu_dycore_tendencies(self.ctx, C.byref(cs.c), buf.ptr))
+        out = buf.download(np.float64, shape)
+        buf.free()
+        return out
+
+    def hydrostatics(self):
+        hs = (self.cfg["ord"] - 1) // 2
+        nzh, nze = self.cfg["nz"] + 2 * hs, self.cfg["nz"] + 1
+        arrs = [np.zeros(nzh) for _ in range(3)] + [np.zeros(nze) for _ in range(3)]
+        check(lib().pu_dycore_hydrostatics(self.ctx, *[a.ctypes.data for a in arrs]))
+        return arrs
+
+    def immersed_masks(self):
+        shape = (self.cfg["nz"], self.cfg["ny"], self.cfg["nx"])
+        any10 = np.zeros(shape, np.uint8)
+        tau = np.zeros(shape, np.int32)
+        check(lib().pu_dycore_immersed_masks(self.ctx, any10.ctypes.data, tau.ctypes.data))
+        return any10, tau
+
+    def diagnostics(self, cs):
+        m, w, a = C.c_double(), C.c_double(), C.c_double()
+        check(lib().pu_diagnostics(self.ctx, C.byref(cs.c), C.byref(m), C.byref(w), C.byref(a)))
+        return m.value, w.value, a.value
+
+    def timer_start(self):
+        check(lib().pu_timer_start(self.ctx))
+
+    def timer_stop(self):
+        ms = C.c_double()
+        check(lib().pu_timer_stop(self.ctx, C.byref(ms)))
+        return ms.value
+
+    def kernel_timing(self, on):
+        check(lib().pu_kernel_timing(self.ctx, int(on)))
+
+    def kernel_time(self, name):
+        ms, n = C.c_double(), C.c_longlong()
+        check(lib().pu_kernel_time(self.ctx, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def destroy(self):
+        if self.ctx:
+            lib().pu_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def case_init(cfg, case_id, seed, cs):
+    pc = to_pu_config(cfg)
+    check(lib().pu_case_init(C.byref(pc), case_id, seed, C.byref(cs.c), cs.immersed.ptr))

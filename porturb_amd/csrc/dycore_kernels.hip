@@ -1,0 +1,642 @@
+// dycore_kernels.hip -- the per-stage kernels of the WENO finite-volume dycore step on gfx950.
+//
+// One SSPRK3 stage (dynamics_rk.h:279-367 with compute_tendencies :498-732) is four launches:
+//   prep      : conserved state -> reconstructed primitive fields [rho', u, v, w, theta', p', q_l]
+//               with the x/y periodic/precursor halos and z wall/periodic halos filled in place
+//               (dynamics_rk.h:547-558 + coupler.h:985-1165 + dynamics_rk.h:736-804);
+//   sweep x/y/z: WENO reconstruction of every field for every cell of a line segment, the
+//               acoustic upwind Riemann flux at each face and the flux divergence, with the face
+//               states exchanged through LDS -- no edge arrays in HBM (:572-728, :808-912);
+//   combine   : gravity/Coriolis sources, the SSPRK stage combination, positive-tracer clipping and
+//               the immersed-boundary relaxation (:308-362, :370-473).
+// Layout: every field SoA [var][z][y][x], x fastest.  The state is double (PU_FP64) or float with a
+// float compensation term for rho and rho*theta (PU_FP32).
+#include "pu_common.h"
+#include "weno_ops.h"
+#include <type_traits>
+
+namespace pu {
+
+struct PrepParams {
+  int nx, ny, nz, hs, hp, ntr, bc_x, bc_y, bc_z;
+  long long n3, nxh, nyh, nzh, nh;
+  double C0, gamma;
+  const double *hy_r, *hy_t, *hy_p;     // cells [nz+2hs]
+};
+
+struct SweepParams {
+  int nx, ny, nz, hs, hp, ntr, bc_along, wall_z, weno_all;
+  long long n3, nxh, nyh, nzh, nh;
+  double rd;                       // 1/dx, 1/dy or 1/dz
+  const double *hy_rc, *hy_tc;     // cells [nz+2hs]
+  const double *hy_re, *hy_te;     // edges [nz+1]
+};
+
+struct CombineParams {
+  int nx, ny, nz, hs, ntr, stage, nrelax, enable_gravity, coriolis;
+  long long n3;
+  double dt, grav, fcor;
+  const double *hy_r, *hy_t;        // cells
+  const double *relax;              // [n3]
+  unsigned int positive_mask;       // bit tr set if tracer tr is positive-definite
+};
+
+__device__ __forceinline__ int wrapi(int i, int n) { i %= n; return i < 0 ? i + n : i; }
+__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+
+// Conserved value as double: FP32 storage adds the compensation term of rho (l=0), rho*theta (l=4).
+template <class ST>
+__device__ __forceinline__ double load_cons(const ST *U, const ST *Ulo, int l, long long id, long long n3) {
+  double v = (double)U[(long long)l * n3 + id];
+  if constexpr (std::is_same<ST, float>::value) {
+    if (l == idR) v += (double)Ulo[id];
+    else if (l == idT) v += (double)Ulo[n3 + id];
+  }
+  return v;
+}
+
+template <class ST>
+__device__ __forceinline__ void store_cons(ST *U, ST *Ulo, int l, long long id, long long n3, double v) {
+  if constexpr (std::is_same<ST, float>::value) {
+    float hi = (float)v;
+    U[(long long)l * n3 + id] = hi;
+    if (l == idR) Ulo[id] = (float)(v - (double)hi);
+    else if (l == idT) Ulo[n3 + id] = (float)(v - (double)hi);
+  } else {
+    U[(long long)l * n3 + id] = v;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// prep: primitive fields with halos (cross-shaped: the halo corners are never read)
+// ------------------------------------------------------------------------------------------------
+template <class ST>
+__global__ __launch_bounds__(256) void k_prep(const ST *__restrict__ U, const ST *__restrict__ Ulo,
+                                              ST *__restrict__ prim, PrepParams P) {
+  int ih = blockIdx.x * blockDim.x + threadIdx.x;
+  int jh = blockIdx.y * blockDim.y + threadIdx.y;
+  int kh = blockIdx.z;
+  if (ih >= P.nxh || jh >= P.nyh || kh >= P.nzh) return;
+  int i = ih - P.hp, j = jh - P.hp, k = kh - P.hp;
+  bool ox = i < 0 || i >= P.nx, oy = j < 0 || j >= P.ny, oz = k < 0 || k >= P.nz;
+  if ((int)ox + (int)oy + (int)oz >= 2) return;
+  int is = i, js = j, ks = k;
+  bool zwall = false;
+  if (ox) is = P.bc_x == PU_BC_PERIODIC ? wrapi(i, P.nx) : clampi(i, 0, P.nx - 1);
+  if (oy) js = P.bc_y == PU_BC_PERIODIC ? wrapi(j, P.ny) : clampi(j, 0, P.ny - 1);
+  if (oz) {
+    if (P.bc_z == PU_BC_PERIODIC) ks = wrapi(k, P.nz);
+    else { ks = clampi(k, 0, P.nz - 1); zwall = true; }
+  }
+  long long sid = ((long long)ks * P.ny + js) * P.nx + is;
+  long long hid = ((long long)kh * P.nyh + jh) * P.nxh + ih;
+  double rho = load_cons(U, Ulo, idR, sid, P.n3);
+  double rt = load_cons(U, Ulo, idT, sid, P.n3);
+  int kc = P.hs + ks;
+  double r_r = 1. / rho;
+  prim[(long long)idP * P.nh + hid] = (ST)(P.C0 * pow(rt, P.gamma) - P.hy_p[kc]);
+  prim[(long long)idR * P.nh + hid] = (ST)(rho - P.hy_r[kc]);
+  prim[(long long)idU * P.nh + hid] = (ST)((double)U[(long long)idU * P.n3 + sid] * r_r);
+  prim[(long long)idV * P.nh + hid] = (ST)((double)U[(long long)idV * P.n3 + sid] * r_r);
+  prim[(long long)idW * P.nh + hid] = zwall ? (ST)0 : (ST)((double)U[(long long)idW * P.n3 + sid] * r_r);
+  prim[(long long)idT * P.nh + hid] = zwall ? (ST)0 : (ST)(rt * r_r - P.hy_t[kc]);
+  for (int tr = 0; tr < P.ntr; tr++)
+    prim[(long long)(NUM_STATE + 1 + tr) * P.nh + hid] =
+        (ST)((double)U[(long long)(NUM_STATE + tr) * P.n3 + sid] * r_r);
+}
+
+// ------------------------------------------------------------------------------------------------
+// WENO limiter (WenoLimiter.h compute_limited_edges / gll_1 / gll_2 / convexify / map_rs)
+// ------------------------------------------------------------------------------------------------
+template <class T> __device__ __forceinline__ T recip(T x) {
+  if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_rcpf(x);
+  else return T(1) / x;
+}
+
+template <class T, int S> __device__ __forceinline__ void convexify(T *w) {
+  T tot = w[0];
+#pragma unroll
+  for (int m = 1; m < S; m++) tot += w[m];
+  if (tot > T(1.e-20)) {
+    T r = recip(tot);
+#pragma unroll
+    for (int m = 0; m < S; m++) w[m] *= r;
+  }
+}
+
+template <class T> __device__ __forceinline__ T map_rs(T w, T d) {     // k=6, m=3, s=2000
+  T dl = w - d;
+  T d2 = dl * dl;
+  T t1 = d2 * d2 * d2;
+  T ww = w * (T(1) - w);
+  T t2 = ww * ww * ww;
+  return d + (t1 * dl) * recip(t1 + T(2000) * t2);
+}
+
+template <class T> __device__ __forceinline__ T map_w(T w, T d) {      // ord 3
+  return w * (d + d * d - T(3) * d * w + w * w) * recip(d * d + (T(1) - T(2) * d) * w);
+}
+
+template <int ORD, class T>
+__device__ __forceinline__ void weno_edges(const T *s, bool do_map, bool imm_L, bool imm_R, T &qL, T &qR) {
+  using O = pu_weno::Ops<ORD>;
+  constexpr int S = O::S;
+  T TV[S];
+  O::template tv<T>(s, TV);
+  convexify<T, S>(TV);
+  if (imm_L || imm_R) {
+    T mx = TV[0];
+#pragma unroll
+    for (int m = 1; m < S; m++) mx = fmax(mx, TV[m]);
+    if (imm_L) TV[S - 1] = mx;
+    if (imm_R) TV[0] = mx;
+  }
+  T rinv[S];
+#pragma unroll
+  for (int m = 0; m < S; m++) rinv[m] = recip(TV[m] * TV[m] + T(1.e-20));
+  T wl[S], wr[S];
+#pragma unroll
+  for (int m = 0; m < S; m++) { wl[m] = T(O::idealL(m)) * rinv[m]; wr[m] = T(O::idealR(m)) * rinv[m]; }
+  convexify<T, S>(wl);
+  convexify<T, S>(wr);
+  if (do_map) {
+#pragma unroll
+    for (int m = 0; m < S; m++) {
+      if constexpr (O::use_map_rs) {
+        wl[m] = map_rs(wl[m], T(O::idealL(m)));
+        wr[m] = map_rs(wr[m], T(O::idealR(m)));
+      } else {
+        wl[m] = map_w(wl[m], T(O::idealL(m)));
+        wr[m] = map_w(wr[m], T(O::idealR(m)));
+      }
+    }
+    convexify<T, S>(wl);
+    convexify<T, S>(wr);
+  }
+  qL = O::template combL<T>(s, wl);
+  qR = O::template combR<T>(s, wr);
+}
+
+// ------------------------------------------------------------------------------------------------
+// sweep along one direction: reconstruction + Riemann + divergence, face states through LDS
+// ------------------------------------------------------------------------------------------------
+template <int DIR> struct SweepShape;
+template <> struct SweepShape<0> { static constexpr int BX = 64, BY = 4, BS = 64; };
+template <> struct SweepShape<1> { static constexpr int BX = 8, BY = 32, BS = 32; };
+template <> struct SweepShape<2> { static constexpr int BX = 8, BY = 32, BS = 32; };
+
+template <int ORD, class T, int DIR>
+__global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ prim, const uint8_t *__restrict__ immf,
+                                               const uint8_t *__restrict__ any10, T *__restrict__ tend,
+                                               SweepParams P) {
+  constexpr int hs = (ORD - 1) / 2;
+  constexpr int BX = SweepShape<DIR>::BX, BY = SweepShape<DIR>::BY, BS = SweepShape<DIR>::BS;
+  constexpr int NC = BS - 2;
+  constexpr int NT = BX * BY;
+  __shared__ T bufA[3][NT];
+  __shared__ T bufB[2][NT];
+
+  const int tx = threadIdx.x, ty = threadIdx.y, lin = ty * BX + tx;
+  const int t = DIR == 0 ? tx : ty;
+  const int nbs = DIR == 0 ? 1 : BX;
+  int i, j, k, c, nal;
+  if (DIR == 0) { c = blockIdx.x * NC - 1 + tx; i = c; j = blockIdx.y * BY + ty; k = blockIdx.z; nal = P.nx; }
+  else if (DIR == 1) { i = blockIdx.x * BX + tx; c = blockIdx.y * NC - 1 + ty; j = c; k = blockIdx.z; nal = P.ny; }
+  else { i = blockIdx.x * BX + tx; j = blockIdx.y; c = blockIdx.z * NC - 1 + ty; k = c; nal = P.nz; }
+  const bool across_ok = DIR == 0 ? (j < P.ny) : (i < P.nx);
+  const bool in_range = across_ok && c >= -1 && c <= nal;
+  const bool periodic = P.bc_along == PU_BC_PERIODIC;
+  const bool ghost = c < 0 || c >= nal;
+  const bool do_recon = in_range && (!ghost || periodic);
+  const bool has_face = in_range && t >= 1 && c >= 0;
+  const bool has_tend = in_range && t >= 1 && t <= NC && c < nal;
+  const bool bface_lo = !periodic && c == 0;
+  const bool bface_hi = !periodic && c == nal;
+  const bool wallz = DIR == 2 && P.wall_z;
+
+  const long long sstride = DIR == 0 ? 1 : (DIR == 1 ? P.nxh : P.nxh * P.nyh);
+  const long long base = ((long long)(k + P.hp) * P.nyh + (j + P.hp)) * P.nxh + (i + P.hp);
+
+  // immersed flags of the stencil -> zero-derivative clamp window, imm_L/imm_R, do_map
+  int lo = 0, hi = ORD - 1;
+  bool imm_L = false, imm_R = false, do_map = true, use_weno = true;
+  if (do_recon) {
+    uint8_t f[ORD];
+#pragma unroll
+    for (int s = 0; s < ORD; s++) f[s] = immf[base + (long long)(s - hs) * sstride];
+    imm_L = f[hs - 1];
+    imm_R = f[hs + 1];
+    if (!f[hs]) {
+#pragma unroll
+      for (int s = ORD - 1; s > hs; s--) if (f[s]) hi = s - 1;
+#pragma unroll
+      for (int s = 0; s < hs; s++) if (f[s]) lo = s + 1;
+    }
+    int ci = i, cj = j, ck = k;
+    if (DIR == 0) ci = wrapi(c, nal); else if (DIR == 1) cj = wrapi(c, nal); else ck = wrapi(c, nal);
+    bool any = any10[((long long)ck * P.ny + cj) * P.nx + ci] != 0;
+    do_map = !any;
+    use_weno = any || P.weno_all;
+  }
+
+  auto recon = [&](int l, bool modify, T &qL, T &qR) {
+    const T *p = prim + (long long)l * P.nh + base - (long long)hs * sstride;
+    T s[ORD];
+    if (modify) {
+#pragma unroll
+      for (int q = 0; q < ORD; q++) s[q] = p[(long long)clampi(q, lo, hi) * sstride];
+    } else {
+#pragma unroll
+      for (int q = 0; q < ORD; q++) s[q] = p[(long long)q * sstride];
+    }
+    if (use_weno) {
+      weno_edges<ORD, T>(s, do_map, imm_L, imm_R, qL, qR);
+    } else {
+      qL = pu_weno::Ops<ORD>::template linL<T>(s);
+      qR = pu_weno::Ops<ORD>::template linR<T>(s);
+    }
+  };
+
+  const long long cid = ((long long)k * P.ny + j) * P.nx + i;
+  const T rd = (T)P.rd;
+  auto accumulate = [&](int fidx, T v) {
+    T *tp = tend + (long long)fidx * P.n3 + cid;
+    if (DIR == 0) *tp = v;
+    else *tp = *tp + v;
+  };
+
+  // hydrostatic add-back: cell values for x/y faces, edge values for z faces (:642-645, :686-689)
+  T hr = 0, ht = 0;
+  if (has_face) {
+    if (DIR == 2) { hr = (T)P.hy_re[c]; ht = (T)P.hy_te[c]; }
+    else { hr = (T)P.hy_rc[hs + k]; ht = (T)P.hy_tc[hs + k]; }
+  }
+
+  // ---- phase A: density, normal velocity, pressure -> acoustic upwinding ----
+  constexpr int ln = 1 + DIR;
+  T qLr = 0, qRr = 0, qLu = 0, qRu = 0, qLp = 0, qRp = 0;
+  if (do_recon) {
+    recon(idR, false, qLr, qRr);
+    recon(ln, false, qLu, qRu);
+    recon(idP, true, qLp, qRp);
+  }
+  bufA[0][lin] = qRr; bufA[1][lin] = qRu; bufA[2][lin] = qRp;
+  __syncthreads();
+  T ru_upw = 0, p_upw = 0;
+  bool upL = true;
+  T Fr = 0, Fu = 0;
+  if (has_face) {
+    T rL = bufA[0][lin - nbs], uL = bufA[1][lin - nbs], pL = bufA[2][lin - nbs];
+    T rR = qLr, uR = qLu, pR = qLp;
+    if (bface_lo) { rL = rR; uL = uR; pL = pR; }
+    if (bface_hi) { rR = rL; uR = uL; pR = pL; }
+    if (wallz && (bface_lo || bface_hi)) { uL = 0; uR = 0; }     // W = 0 on both sides at walls
+    rL += hr; rR += hr;
+    const T cs = T(350.);
+    T ruL = uL * rL, ruR = uR * rR;
+    p_upw = T(0.5) * (pL + pR - cs * (ruR - ruL));
+    ru_upw = T(0.5) * (ruL + ruR - (pR - pL) / cs);
+    upL = ru_upw > 0;
+    Fr = ru_upw;
+    Fu = ru_upw * (upL ? uL : uR) + p_upw;
+  }
+  bufB[0][lin] = Fr; bufB[1][lin] = Fu;
+  __syncthreads();
+  if (has_tend) {
+    accumulate(idR, -(bufB[0][lin + nbs] - bufB[0][lin]) * rd);
+    accumulate(ln, -(bufB[1][lin + nbs] - bufB[1][lin]) * rd);
+  }
+
+  // ---- phase B: the advected fields, one at a time ----
+  constexpr int lt1 = DIR == 0 ? idV : idU;
+  constexpr int lt2 = DIR == 2 ? idV : idW;
+  const int nB = 3 + P.ntr;
+  for (int b = 0; b < nB; b++) {
+    int l, fidx;
+    bool modify = false;
+    if (b == 0) { l = lt1; fidx = lt1; modify = true; }
+    else if (b == 1) { l = lt2; fidx = lt2; modify = true; }
+    else if (b == 2) { l = idT; fidx = idT; }
+    else { l = NUM_STATE + 1 + (b - 3); fidx = NUM_STATE + (b - 3); }
+    T qL = 0, qR = 0;
+    if (do_recon) recon(l, modify, qL, qR);
+    bufA[0][lin] = qR;
+    __syncthreads();
+    T F = 0;
+    if (has_face) {
+      T vL = bufA[0][lin - nbs], vR = qL;
+      if (bface_lo) vL = vR;
+      if (bface_hi) vR = vL;
+      if (wallz && (bface_lo || bface_hi) && l == idT) { vL = 0; vR = 0; }
+      if (l == idT) { vL += ht; vR += ht; }
+      F = ru_upw * (upL ? vL : vR);
+    }
+    bufB[0][lin] = F;
+    __syncthreads();
+    if (has_tend) accumulate(fidx, -(bufB[0][lin + nbs] - bufB[0][lin]) * rd);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// combine: sources + SSPRK3 stage + clip + immersed relaxation
+// ------------------------------------------------------------------------------------------------
+template <class ST>
+__global__ __launch_bounds__(256) void k_combine(ST *__restrict__ U, ST *__restrict__ Ulo, ST *__restrict__ Us,
+                                                 ST *__restrict__ Uslo, const ST *__restrict__ tend,
+                                                 CombineParams P) {
+  long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= P.n3) return;
+  int k = (int)(id / ((long long)P.nx * P.ny));
+  const int nv = NUM_STATE + P.ntr;
+  // stage input: U for stage 0, Us otherwise
+  const ST *Sin = P.stage == 0 ? U : Us;
+  const ST *Sinlo = P.stage == 0 ? Ulo : Uslo;
+  double rho_in = load_cons(Sin, Sinlo, idR, id, P.n3);
+  double mult = P.relax[id];
+  double tr_r = P.hy_r[P.hs + k];
+  double tr_rt = P.hy_r[P.hs + k] * P.hy_t[P.hs + k];
+  for (int l = 0; l < nv; l++) {
+    double tn = (double)tend[(long long)l * P.n3 + id];
+    if (l == idW && P.enable_gravity) tn += -P.grav * (rho_in - P.hy_r[P.hs + k]);
+    if (P.coriolis && l == idU) tn += P.fcor * load_cons(Sin, Sinlo, idV, id, P.n3);
+    if (P.coriolis && l == idV) tn -= P.fcor * load_cons(Sin, Sinlo, idU, id, P.n3);
+    double un = load_cons(U, Ulo, l, id, P.n3);
+    double v;
+    if (P.stage == 0) {
+      v = un + P.dt * tn;
+    } else if (P.stage == 1) {
+      double us = load_cons(Us, Uslo, l, id, P.n3);
+      v = (3. / 4.) * un + (1. / 4.) * us + (1. / 4.) * P.dt * tn;
+    } else {
+      double us = load_cons(Us, Uslo, l, id, P.n3);
+      v = (1. / 3.) * un + (2. / 3.) * us + (2. / 3.) * P.dt * tn;
+    }
+    bool positive = l >= NUM_STATE && ((P.positive_mask >> (l - NUM_STATE)) & 1u);
+    if (P.stage == 2 && positive) v = fmax(0., v);
+    double target = l == idR ? tr_r : (l == idT ? tr_rt : 0.);
+    for (int r = 0; r < P.nrelax; r++) {
+      v = v + (target - v) * mult;
+      if (positive) v = fmax(0., v);
+    }
+    if (P.stage == 2) store_cons(U, Ulo, l, id, P.n3, v);
+    else store_cons(Us, Uslo, l, id, P.n3, v);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// coupler <-> dynamics conversions (dynamics_rk.h:1251-1352), with the first relaxation fused
+// ------------------------------------------------------------------------------------------------
+struct ConvParams {
+  int nx, ny, nz, hs, ntr, idWV, relax;
+  long long n3;
+  double R_d, R_v, C0, gamma;
+  unsigned int adds_mass_mask, positive_mask;
+  const double *hy_r, *hy_t, *relax_mult;
+};
+
+template <class ST>
+__global__ __launch_bounds__(256) void k_c2d(const double *__restrict__ rd_, const double *__restrict__ uu,
+                                             const double *__restrict__ vv, const double *__restrict__ ww,
+                                             const double *__restrict__ tt, TracerPtrs trc, ST *U, ST *Ulo,
+                                             ConvParams P) {
+  long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= P.n3) return;
+  int k = (int)(id / ((long long)P.nx * P.ny));
+  double rho_d = rd_[id], u = uu[id], v = vv[id], w = ww[id], temp = tt[id];
+  double rho_v = trc.p[P.idWV][id];
+  double press = rho_d * P.R_d * temp + rho_v * P.R_v * temp;
+  double rho = rho_d;
+  for (int tr = 0; tr < P.ntr; tr++) if ((P.adds_mass_mask >> tr) & 1u) rho += trc.p[tr][id];
+  double theta = pow(press / P.C0, 1. / P.gamma) / rho;
+  double vals[NUM_STATE] = {rho, rho * u, rho * v, rho * w, rho * theta};
+  double mult = P.relax ? P.relax_mult[id] : 0.;
+  double tr_r = P.hy_r[P.hs + k], tr_rt = P.hy_r[P.hs + k] * P.hy_t[P.hs + k];
+  for (int l = 0; l < NUM_STATE; l++) {
+    double x = vals[l];
+    if (P.relax) {
+      double target = l == idR ? tr_r : (l == idT ? tr_rt : 0.);
+      x = x + (target - x) * mult;
+    }
+    store_cons(U, Ulo, l, id, P.n3, x);
+  }
+  for (int tr = 0; tr < P.ntr; tr++) {
+    double x = trc.p[tr][id];
+    if (P.relax) {
+      x = x + (0. - x) * mult;
+      if ((P.positive_mask >> tr) & 1u) x = fmax(0., x);
+    }
+    store_cons(U, Ulo, NUM_STATE + tr, id, P.n3, x);
+  }
+}
+
+template <class ST>
+__global__ __launch_bounds__(256) void k_d2c(const ST *__restrict__ U, const ST *__restrict__ Ulo,
+                                             double *rd_, double *uu, double *vv, double *ww, double *tt,
+                                             TracerPtrs trc, ConvParams P) {
+  long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= P.n3) return;
+  double rho = load_cons(U, Ulo, idR, id, P.n3);
+  double u = load_cons(U, Ulo, idU, id, P.n3) / rho;
+  double v = load_cons(U, Ulo, idV, id, P.n3) / rho;
+  double w = load_cons(U, Ulo, idW, id, P.n3) / rho;
+  double theta = load_cons(U, Ulo, idT, id, P.n3) / rho;
+  double press = P.C0 * pow(rho * theta, P.gamma);
+  double rho_v = load_cons(U, Ulo, NUM_STATE + P.idWV, id, P.n3);
+  double rho_d = rho;
+  for (int tr = 0; tr < P.ntr; tr++)
+    if ((P.adds_mass_mask >> tr) & 1u) rho_d -= load_cons(U, Ulo, NUM_STATE + tr, id, P.n3);
+  double temp = press / (rho_d * P.R_d + rho_v * P.R_v);
+  rd_[id] = rho_d; uu[id] = u; vv[id] = v; ww[id] = w; tt[id] = temp;
+  for (int tr = 0; tr < P.ntr; tr++) trc.p[tr][id] = load_cons(U, Ulo, NUM_STATE + tr, id, P.n3);
+}
+
+template <class ST>
+__global__ __launch_bounds__(256) void k_tend_export(const ST *__restrict__ tend, const ST *U, const ST *Ulo,
+                                                     double *out, CombineParams P) {
+  long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= P.n3) return;
+  int k = (int)(id / ((long long)P.nx * P.ny));
+  const int nv = NUM_STATE + P.ntr;
+  double rho_in = load_cons(U, Ulo, idR, id, P.n3);
+  for (int l = 0; l < nv; l++) {
+    double tn = (double)tend[(long long)l * P.n3 + id];
+    if (l == idW && P.enable_gravity) tn += -P.grav * (rho_in - P.hy_r[P.hs + k]);
+    if (P.coriolis && l == idU) tn += P.fcor * load_cons(U, Ulo, idV, id, P.n3);
+    if (P.coriolis && l == idV) tn -= P.fcor * load_cons(U, Ulo, idU, id, P.n3);
+    out[(long long)l * P.n3 + id] = tn;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------------------
+static inline const double *hy_cells(pu_ctx *ctx, int w) { return ctx->d_hy + (size_t)w * (ctx->g.nz + 2 * ctx->g.hs); }
+static inline const double *hy_edges(pu_ctx *ctx, int w) {
+  return ctx->d_hy + (size_t)3 * (ctx->g.nz + 2 * ctx->g.hs) + (size_t)w * (ctx->g.nz + 1);
+}
+
+template <class ST>
+static int prep_t(pu_ctx *ctx, const void *U, const void *Ulo, hipStream_t s) {
+  const pu::Grid &g = ctx->g;
+  PrepParams P;
+  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ntr = ctx->cfg.num_tracers;
+  P.bc_x = ctx->cfg.bc_x; P.bc_y = ctx->cfg.bc_y; P.bc_z = ctx->cfg.bc_z;
+  P.n3 = g.n3(); P.nxh = g.nxh; P.nyh = g.nyh; P.nzh = g.nzh; P.nh = g.nh();
+  P.C0 = ctx->cfg.C0; P.gamma = ctx->cfg.gamma_d;
+  P.hy_r = hy_cells(ctx, 0); P.hy_t = hy_cells(ctx, 1); P.hy_p = hy_cells(ctx, 2);
+  dim3 blk(64, 4, 1), grd((unsigned)((g.nxh + 63) / 64), (unsigned)((g.nyh + 3) / 4), (unsigned)g.nzh);
+  LaunchTimer lt(ctx, "prep");
+  hipLaunchKernelGGL(k_prep<ST>, grd, blk, 0, s, (const ST *)U, (const ST *)Ulo, (ST *)ctx->d_prim, P);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
+int launch_prep(pu_ctx *ctx, const void *U, const void *Ulo, hipStream_t s) {
+  return ctx->cfg.precision == PU_FP32 ? prep_t<float>(ctx, U, Ulo, s) : prep_t<double>(ctx, U, Ulo, s);
+}
+
+template <int ORD, class T, int DIR>
+static int sweep_one(pu_ctx *ctx, hipStream_t s) {
+  const pu::Grid &g = ctx->g;
+  SweepParams P;
+  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ntr = ctx->cfg.num_tracers;
+  P.bc_along = DIR == 0 ? ctx->cfg.bc_x : (DIR == 1 ? ctx->cfg.bc_y : ctx->cfg.bc_z);
+  if (DIR == 2 && P.bc_along == PU_BC_SOLID_WALL) P.bc_along = PU_BC_PRECURSOR;   // non-periodic
+  P.wall_z = ctx->cfg.bc_z == PU_BC_SOLID_WALL;
+  P.weno_all = ctx->cfg.weno_all;
+  P.n3 = g.n3(); P.nxh = g.nxh; P.nyh = g.nyh; P.nzh = g.nzh; P.nh = g.nh();
+  double d = DIR == 0 ? ctx->cfg.xlen / g.nx : (DIR == 1 ? ctx->cfg.ylen / g.ny : ctx->cfg.zlen / g.nz);
+  P.rd = 1. / d;
+  P.hy_rc = hy_cells(ctx, 0); P.hy_tc = hy_cells(ctx, 1);
+  P.hy_re = hy_edges(ctx, 0); P.hy_te = hy_edges(ctx, 1);
+  constexpr int BX = SweepShape<DIR>::BX, BY = SweepShape<DIR>::BY, NC = SweepShape<DIR>::BS - 2;
+  dim3 blk(BX, BY, 1), grd;
+  if (DIR == 0) grd = dim3((unsigned)((g.nx + NC - 1) / NC), (unsigned)((g.ny + BY - 1) / BY), (unsigned)g.nz);
+  else if (DIR == 1) grd = dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)((g.ny + NC - 1) / NC), (unsigned)g.nz);
+  else grd = dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)g.ny, (unsigned)((g.nz + NC - 1) / NC));
+  // A segment's last thread is the neighbour cell past the end; make sure the grid reaches cell n.
+  static const char *names[3] = {"weno_x", "weno_y", "weno_z"};
+  LaunchTimer lt(ctx, names[DIR]);
+  hipLaunchKernelGGL((k_sweep<ORD, T, DIR>), grd, blk, 0, s, (const T *)ctx->d_prim, ctx->d_imm_flag,
+                     ctx->d_any10, (T *)ctx->d_tend, P);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
+template <int ORD, class T>
+static int sweeps_t(pu_ctx *ctx, hipStream_t s) {
+  int rc;
+  if ((rc = sweep_one<ORD, T, 0>(ctx, s))) return rc;
+  if (ctx->g.ny > 1 && (rc = sweep_one<ORD, T, 1>(ctx, s))) return rc;
+  if ((rc = sweep_one<ORD, T, 2>(ctx, s))) return rc;
+  return PU_OK;
+}
+
+int launch_sweeps(pu_ctx *ctx, hipStream_t s) {
+  bool f32 = ctx->cfg.precision == PU_FP32;
+  switch (ctx->cfg.ord) {
+    case 3: return f32 ? sweeps_t<3, float>(ctx, s) : sweeps_t<3, double>(ctx, s);
+    case 5: return f32 ? sweeps_t<5, float>(ctx, s) : sweeps_t<5, double>(ctx, s);
+    case 7: return f32 ? sweeps_t<7, float>(ctx, s) : sweeps_t<7, double>(ctx, s);
+    case 9: return f32 ? sweeps_t<9, float>(ctx, s) : sweeps_t<9, double>(ctx, s);
+    case 11: return f32 ? sweeps_t<11, float>(ctx, s) : sweeps_t<11, double>(ctx, s);
+  }
+  return fail(PU_ERR_UNSUPPORTED, "unsupported ord");
+}
+
+static CombineParams combine_params(pu_ctx *ctx) {
+  CombineParams P;
+  P.nx = ctx->g.nx; P.ny = ctx->g.ny; P.nz = ctx->g.nz; P.hs = ctx->g.hs; P.ntr = ctx->cfg.num_tracers;
+  P.n3 = ctx->g.n3();
+  P.enable_gravity = ctx->cfg.enable_gravity;
+  P.grav = ctx->cfg.grav;
+  P.coriolis = ctx->cfg.latitude != 0 && ctx->g.ny > 1;
+  P.fcor = 2 * 7.2921e-5 * sin(ctx->cfg.latitude / 180 * M_PI);
+  P.hy_r = hy_cells(ctx, 0); P.hy_t = hy_cells(ctx, 1);
+  P.relax = ctx->d_relax;
+  P.positive_mask = 0;
+  for (int tr = 0; tr < ctx->cfg.num_tracers; tr++) if (ctx->cfg.tracer_positive[tr]) P.positive_mask |= 1u << tr;
+  P.stage = 0; P.nrelax = 1; P.dt = 0;
+  return P;
+}
+
+template <class ST>
+static int combine_t(pu_ctx *ctx, int stage, double dt, int nrelax, hipStream_t s) {
+  CombineParams P = combine_params(ctx);
+  P.stage = stage; P.nrelax = nrelax; P.dt = dt;
+  long long n = ctx->g.n3();
+  LaunchTimer lt(ctx, "combine");
+  hipLaunchKernelGGL(k_combine<ST>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (ST *)ctx->d_U,
+                     (ST *)ctx->d_Ulo, (ST *)ctx->d_Us, (ST *)ctx->d_Uslo, (const ST *)ctx->d_tend, P);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
+int launch_combine(pu_ctx *ctx, int stage, double dt, int nrelax, hipStream_t s) {
+  return ctx->cfg.precision == PU_FP32 ? combine_t<float>(ctx, stage, dt, nrelax, s)
+                                       : combine_t<double>(ctx, stage, dt, nrelax, s);
+}
+
+static ConvParams conv_params(pu_ctx *ctx) {
+  ConvParams P;
+  P.nx = ctx->g.nx; P.ny = ctx->g.ny; P.nz = ctx->g.nz; P.hs = ctx->g.hs; P.ntr = ctx->cfg.num_tracers;
+  P.idWV = ctx->cfg.idWV; P.n3 = ctx->g.n3();
+  P.R_d = ctx->cfg.R_d; P.R_v = ctx->cfg.R_v; P.C0 = ctx->cfg.C0; P.gamma = ctx->cfg.gamma_d;
+  P.adds_mass_mask = 0; P.positive_mask = 0;
+  for (int tr = 0; tr < ctx->cfg.num_tracers; tr++) {
+    if (ctx->cfg.tracer_adds_mass[tr]) P.adds_mass_mask |= 1u << tr;
+    if (ctx->cfg.tracer_positive[tr]) P.positive_mask |= 1u << tr;
+  }
+  P.hy_r = hy_cells(ctx, 0); P.hy_t = hy_cells(ctx, 1); P.relax_mult = ctx->d_relax;
+  P.relax = 0;
+  return P;
+}
+
+int launch_convert_c2d(pu_ctx *ctx, const pu_coupler_state *cs, int relax, hipStream_t s) {
+  ConvParams P = conv_params(ctx);
+  P.relax = relax;
+  TracerPtrs tp;
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++) tp.p[tr] = tr < ctx->cfg.num_tracers ? cs->tracers[tr] : nullptr;
+  long long n = ctx->g.n3();
+  dim3 grd((unsigned)((n + 255) / 256)), blk(256);
+  if (ctx->cfg.precision == PU_FP32)
+    hipLaunchKernelGGL(k_c2d<float>, grd, blk, 0, s, cs->density_dry, cs->uvel, cs->vvel, cs->wvel, cs->temp, tp,
+                       (float *)ctx->d_U, (float *)ctx->d_Ulo, P);
+  else
+    hipLaunchKernelGGL(k_c2d<double>, grd, blk, 0, s, cs->density_dry, cs->uvel, cs->vvel, cs->wvel, cs->temp, tp,
+                       (double *)ctx->d_U, (double *)ctx->d_Ulo, P);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
+int launch_convert_d2c(pu_ctx *ctx, pu_coupler_state *cs, hipStream_t s) {
+  ConvParams P = conv_params(ctx);
+  TracerPtrs tp;
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++) tp.p[tr] = tr < ctx->cfg.num_tracers ? cs->tracers[tr] : nullptr;
+  long long n = ctx->g.n3();
+  dim3 grd((unsigned)((n + 255) / 256)), blk(256);
+  if (ctx->cfg.precision == PU_FP32)
+    hipLaunchKernelGGL(k_d2c<float>, grd, blk, 0, s, (const float *)ctx->d_U, (const float *)ctx->d_Ulo,
+                       cs->density_dry, cs->uvel, cs->vvel, cs->wvel, cs->temp, tp, P);
+  else
+    hipLaunchKernelGGL(k_d2c<double>, grd, blk, 0, s, (const double *)ctx->d_U, (const double *)ctx->d_Ulo,
+                       cs->density_dry, cs->uvel, cs->vvel, cs->wvel, cs->temp, tp, P);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
+int launch_tend_export(pu_ctx *ctx, double *out, hipStream_t s) {
+  CombineParams P = combine_params(ctx);
+  long long n = ctx->g.n3();
+  dim3 grd((unsigned)((n + 255) / 256)), blk(256);
+  if (ctx->cfg.precision == PU_FP32)
+    hipLaunchKernelGGL(k_tend_export<float>, grd, blk, 0, s, (const float *)ctx->d_tend, (const float *)ctx->d_U,
+                       (const float *)ctx->d_Ulo, out, P);
+  else
+    hipLaunchKernelGGL(k_tend_export<double>, grd, blk, 0, s, (const double *)ctx->d_tend, (const double *)ctx->d_U,
+                       (const double *)ctx->d_Ulo, out, P);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
+}  // namespace pu

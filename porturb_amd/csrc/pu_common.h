@@ -1,0 +1,119 @@
+// pu_common.h -- internal definitions shared by the HIP kernels and the C ABI of libporturb.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <map>
+#include <vector>
+#include "porturb.h"
+
+namespace pu {
+
+// Conserved-state indices (dynamics_rk.h:33-38) and the pressure slot of the reconstructed
+// fields (dynamics_rk.h:496).
+constexpr int idR = 0, idU = 1, idV = 2, idW = 3, idT = 4, idP = 5, NUM_STATE = 5;
+
+// Reconstruction halo of the primitive-field array: hs cells for the stencils of interior cells
+// plus one more for the neighbour-cell reconstruction each line segment needs at its ends.
+__host__ __device__ constexpr int prim_halo(int hs) { return hs + 1; }
+
+struct Grid {
+  int nx, ny, nz, hs, hp;          // hp = prim_halo(hs)
+  long long nxh, nyh, nzh;         // halo'd extents of the primitive-field array
+  long long n3() const { return (long long)nx * ny * nz; }
+  long long nh() const { return nxh * nyh * nzh; }
+};
+
+// Timing of one kernel family with HIP events around each launch.
+struct KernelTimer {
+  double total_ms = 0;
+  long long count = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+}  // namespace pu
+
+struct pu_ctx {
+  pu_config cfg;
+  hipStream_t stream = nullptr;
+  pu::Grid g{};
+  int nv = 0;                      // 5 + num_tracers  (conserved variables)
+  int nf = 0;                      // nv + 1           (reconstructed fields incl. pressure)
+  bool dycore_ready = false;
+  // hydrostatic background, FP64 (host + device copies)
+  std::vector<double> hy_cells[3];     // dens, theta, pressure: [nz+2hs]
+  std::vector<double> hy_edges[3];     // [nz+1]
+  double *d_hy = nullptr;              // device: [3*(nz+2hs)] cells then [3*(nz+1)] edges
+  // immersed data
+  uint8_t *d_imm_flag = nullptr;       // (prop_halos > 0), halo'd with the primitive halo
+  uint8_t *d_any10 = nullptr;          // dycore_any_immersed10 [n3]
+  int     *d_tau = nullptr;            // dycore_immersed_tau [n3]
+  double  *d_relax = nullptr;          // pow(prop, immersed_power)/tau [n3]
+  uint8_t *d_imm_les = nullptr;        // immersed_proportion_halos (hs=1) as 0 / partial / 1 codes
+  double  *d_imm_les_val = nullptr;    // immersed_proportion_halos values [nz+2][ny+2][nx+2]
+  // dycore work arrays (element type follows cfg.precision)
+  void *d_U = nullptr, *d_Us = nullptr;        // [nv][n3]
+  void *d_Ulo = nullptr, *d_Uslo = nullptr;    // FP32 only: compensation of rho, rho*theta [2][n3]
+  void *d_tend = nullptr;                      // [nv][n3]
+  void *d_prim = nullptr;                      // [nf][nzh][nyh][nxh]
+  // LES work arrays
+  double *d_les = nullptr;                     // primitive closure state, halo 1
+  // reductions
+  double *d_red = nullptr;                     // scratch for reductions
+  // timing
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  bool kernel_timing = false;
+  std::map<std::string, pu::KernelTimer> timers;
+};
+
+namespace pu {
+
+void set_error(const std::string &msg);
+int  fail(int code, const std::string &msg);
+
+#define PU_HIP_CHECK(expr)                                                                  \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      return ::pu::fail(PU_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+  } while (0)
+
+// Tracer pointer bundles passed by value to kernels.
+struct TracerPtrs { double *p[PU_MAX_TRACERS]; };
+struct TracerPtrsC { const double *p[PU_MAX_TRACERS]; };
+
+// Kernel-timing helpers: record events around a launch when timing is enabled.
+struct LaunchTimer {
+  pu_ctx *ctx; const char *name; hipEvent_t a = nullptr, b = nullptr;
+  LaunchTimer(pu_ctx *c, const char *n) : ctx(c), name(n) {
+    if (ctx->kernel_timing) {
+      hipEventCreate(&a); hipEventCreate(&b);
+      hipEventRecord(a, ctx->stream);
+    }
+  }
+  ~LaunchTimer() {
+    if (ctx->kernel_timing) {
+      hipEventRecord(b, ctx->stream);
+      ctx->timers[name].pending.emplace_back(a, b);
+    }
+  }
+};
+
+void collect_timers(pu_ctx *ctx);
+
+// Launchers implemented in the kernel translation units.
+int launch_prep(pu_ctx *ctx, const void *U, const void *Ulo, hipStream_t s);
+int launch_sweeps(pu_ctx *ctx, hipStream_t s);
+int launch_combine(pu_ctx *ctx, int stage, double dt, int nrelax, hipStream_t s);
+int launch_convert_c2d(pu_ctx *ctx, const pu_coupler_state *cs, int relax, hipStream_t s);
+int launch_convert_d2c(pu_ctx *ctx, pu_coupler_state *cs, hipStream_t s);
+int launch_tend_export(pu_ctx *ctx, double *tend, hipStream_t s);
+int dycore_init_device(pu_ctx *ctx, const pu_coupler_state *cs);
+int les_apply_device(pu_ctx *ctx, pu_coupler_state *cs, double dt);
+int diagnostics_device(pu_ctx *ctx, const pu_coupler_state *cs, double *mass, double *maxw, double *maxabsw);
+int case_init_device(const pu_config *cfg, int case_id, unsigned long long seed, pu_coupler_state *cs,
+                     double *imm, hipStream_t s);
+
+}  // namespace pu

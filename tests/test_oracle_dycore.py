@@ -1,0 +1,170 @@
+"""Invariant and closed-form tests of the oracle's dycore step (dynamics_rk.h restated in C).
+
+What pins what:
+* rest state: an isentropic atmosphere at rest is the hydrostatic background itself, so every
+  perturbation, flux and source is zero and the step leaves it unchanged (hydrostatic means,
+  p', gravity term, wall halos).
+* hydrostatic halos: the isentropic extrapolation (dynamics_rk.h:985-1008) equals the closed-form
+  constant-theta profile (profiles.h:12-21) at the halo heights.
+* mass: with x/y periodic, z walls and no immersed cells, sum(rho) is conserved to round-off
+  (flux form, zero wall mass flux).
+* Galilean: a uniform horizontal flow without gravity has zero tendencies (w = 0: the hydrostatic
+  edge values near the z boundaries are extrapolated with gravity even when it is switched off,
+  dynamics_rk.h:985-1008, so a vertical mass flux would not be uniform there).
+* SSPRK3: time_step equals the Shu-Osher composition of the oracle's own compute_tendencies.
+* conversions: coupler -> dynamics -> coupler is the identity.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_py
+from porturb_amd import PU_BC_PERIODIC, PU_BC_PRECURSOR, physical_constants
+from tests.helpers import random_state, small_config, isentropic, WV, TKE, P1, rel_norms
+
+
+def rest_state(cfg):
+    nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
+    dz = cfg["zlen"] / nz
+    c = physical_constants()
+    z = (np.arange(nz) + 0.5) * dz
+    rho = np.broadcast_to(isentropic(z)[:, None, None], (nz, ny, nx)).copy()
+    st = np.zeros((5 + cfg["num_tracers"], nz, ny, nx))
+    st[0] = rho
+    st[4] = c["C0"] * (rho * 300.) ** c["gamma_d"] / (rho * c["R_d"])
+    return st, np.zeros((nz, ny, nx))
+
+
+@pytest.mark.parametrize("ord_", (3, 5, 9))
+def test_rest_state_is_steady(ord_):
+    cfg = small_config(ord=ord_, tracers=[WV])
+    st, imm = rest_state(cfg)
+    d = oracle_py.Dycore(cfg, st, imm)
+    tend = d.tendencies(st)
+    assert np.abs(tend).max() < 1e-10
+    s2 = st.copy()
+    for _ in range(3):
+        d.time_step(s2, d.compute_time_step())
+    assert np.abs(s2 - st).max() / np.abs(st).max() < 1e-13
+
+
+@pytest.mark.parametrize("ord_", (5, 9))
+def test_hydrostatic_halos_follow_isentrope(ord_):
+    cfg = small_config(ord=ord_, tracers=[WV])
+    st, imm = rest_state(cfg)
+    d = oracle_py.Dycore(cfg, st, imm)
+    rc, tc, pc, re_, te, pe = d.hydrostatics()
+    hs = (ord_ - 1) // 2
+    dz = cfg["zlen"] / cfg["nz"]
+    z = (np.arange(cfg["nz"] + 2 * hs) - hs + 0.5) * dz
+    np.testing.assert_allclose(rc, isentropic(z), rtol=1e-12)
+    np.testing.assert_allclose(tc, 300., rtol=1e-13)
+    # edges: 4th-order log interpolation of the cells
+    ze = np.arange(cfg["nz"] + 1) * dz
+    np.testing.assert_allclose(re_, isentropic(ze), rtol=1e-6)
+
+
+@pytest.mark.parametrize("ord_", (5, 9))
+def test_mass_is_conserved(ord_):
+    cfg = small_config(ord=ord_, tracers=[WV, P1])
+    st, imm = random_state(cfg, seed=3)
+    d = oracle_py.Dycore(cfg, st, imm)
+    m0 = oracle_py.total_mass(cfg, st)
+    s2 = st.copy()
+    for _ in range(4):
+        d.time_step(s2, d.compute_time_step())
+    m1 = oracle_py.total_mass(cfg, s2)
+    assert abs(m1 - m0) / m0 < 1e-14
+
+
+def test_uniform_flow_is_galilean_invariant():
+    cfg = small_config(ord=5, tracers=[WV], bc_z=PU_BC_PERIODIC, enable_gravity=0)
+    nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
+    c = physical_constants()
+    st = np.zeros((6, nz, ny, nx))
+    st[0] = 1.1
+    st[1], st[2], st[3] = 7., -3., 0.
+    st[4] = c["C0"] * (1.1 * 300.) ** c["gamma_d"] / (1.1 * c["R_d"])
+    d = oracle_py.Dycore(cfg, st, np.zeros((nz, ny, nx)))
+    assert np.abs(d.tendencies(st)).max() < 1e-9
+
+
+def _cons_from_coupler(cfg, cp):
+    c = physical_constants()
+    ntr = cfg["num_tracers"]
+    rho_d, u, v, w, T = cp[:5]
+    trc = cp[5:]
+    rv = trc[cfg["idWV"]]
+    press = rho_d * c["R_d"] * T + rv * c["R_v"] * T
+    rho = rho_d + sum(trc[t] for t in range(ntr) if cfg["tracers"][t][2])
+    theta = (press / c["C0"]) ** (1 / c["gamma_d"]) / rho
+    return np.concatenate([np.stack([rho, rho * u, rho * v, rho * w, rho * theta]), trc])
+
+
+def _coupler_from_cons(cfg, U):
+    c = physical_constants()
+    ntr = cfg["num_tracers"]
+    rho = U[0]
+    trc = U[5:]
+    u, v, w, theta = U[1] / rho, U[2] / rho, U[3] / rho, U[4] / rho
+    press = c["C0"] * (rho * theta) ** c["gamma_d"]
+    rd = rho - sum(trc[t] for t in range(ntr) if cfg["tracers"][t][2])
+    T = press / (rd * c["R_d"] + trc[cfg["idWV"]] * c["R_v"])
+    return np.concatenate([np.stack([rd, u, v, w, T]), trc])
+
+
+def test_ssprk3_is_shu_osher_composition():
+    cfg = small_config(ord=5, tracers=[WV, P1])
+    # keep tracers well away from zero so the positivity clip stays inactive
+    st, imm = random_state(cfg, seed=5)
+    st[5 + 1] += 0.5 * st[0]
+    d = oracle_py.Dycore(cfg, st, imm)
+    dt = d.compute_time_step()
+    L = lambda U: d.tendencies(_coupler_from_cons(cfg, U))
+    U0 = _cons_from_coupler(cfg, st)
+    U1 = U0 + dt * L(U0)
+    U2 = 0.75 * U0 + 0.25 * U1 + 0.25 * dt * L(U1)
+    U3 = U0 / 3. + 2. / 3. * U2 + 2. / 3. * dt * L(U2)
+    s2 = st.copy()
+    d.time_step(s2, dt)
+    ref = _coupler_from_cons(cfg, U3)
+    assert rel_norms(s2, ref)[:, 2].max() < 1e-12
+
+
+def test_conversion_round_trip():
+    cfg = small_config(ord=5, tracers=[WV, TKE, P1])
+    st, imm = random_state(cfg, seed=9)
+    back = _coupler_from_cons(cfg, _cons_from_coupler(cfg, st))
+    assert rel_norms(back, st)[:, 2].max() < 1e-14
+
+
+def test_ncycles_split_the_physics_step():
+    """time_step(dt_phys) with dt_phys = 2*dt_dyn runs two SSPRK3 cycles of dt_dyn."""
+    cfg = small_config(ord=5, tracers=[WV])
+    st, imm = random_state(cfg, seed=11)
+    d = oracle_py.Dycore(cfg, st, imm)
+    dt = d.compute_time_step()
+    a = st.copy(); d.time_step(a, 2 * dt)
+    b = st.copy(); d.time_step(b, dt); d.time_step(b, dt)
+    assert rel_norms(a, b)[:, 2].max() < 1e-11
+
+
+def test_immersed_masks():
+    """any_immersed10 is the 21^3 box dilation of (prop > 0) with z halos 0; tau is 8 next to
+    open cells and smaller deep inside a building."""
+    cfg = small_config(nx=24, ny=24, nz=16, ord=5, tracers=[WV])
+    st, _ = random_state(cfg, seed=1)
+    imm = np.zeros((16, 24, 24))
+    imm[:8, 4:14, 4:14] = 1.
+    d = oracle_py.Dycore(cfg, st, imm)
+    any10, tau = d.immersed_masks()
+    # brute-force dilation
+    ref = np.zeros_like(any10)
+    for k in range(16):
+        for j in range(24):
+            for i in range(24):
+                ks = slice(max(0, k - 10), min(16, k + 11))
+                jj = [(j + o) % 24 for o in range(-10, 11)]
+                ii = [(i + o) % 24 for o in range(-10, 11)]
+                ref[k, j, i] = (imm[ks][:, jj][:, :, ii] > 0).any()
+    assert (any10 == ref).all()
+    assert tau[0, 0, 0] == 8 and tau[3, 9, 9] == 1 and tau[5, 9, 9] == 2 and tau[6, 9, 9] == 4
