@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdio>
+#include <algorithm>
 
 namespace pu {
 
@@ -105,7 +106,7 @@ int pu_create(const pu_config *cfg, void *stream, pu_ctx **out) {
       (rc = alloc(&c->d_relax, n3 * sizeof(double))) || (rc = alloc(&c->d_imm_les_val, nl * sizeof(double))) ||
       (rc = alloc(&c->d_U, es * c->nv * n3)) || (rc = alloc(&c->d_Us, es * c->nv * n3)) ||
       (rc = alloc(&c->d_tend, es * c->nv * n3)) || (rc = alloc(&c->d_prim, es * c->nf * nh)) ||
-      (rc = alloc(&c->d_red, sizeof(double) * 4096))) {
+      (rc = alloc(&c->d_red, sizeof(double) * (size_t)std::max(4096, 3 * g.nz + 16)))) {
     free_ctx(c); delete c; return rc;
   }
   if (cfg->precision == PU_FP32) {

@@ -137,6 +137,67 @@ template <class T> __device__ __forceinline__ T map_w(T w, T d) {      // ord 3
   return w * (d + d * d - T(3) * d * w + w * w) * recip(d * d + (T(1) - T(2) * d) * w);
 }
 
+// P[m] = prod_{j != m} D[j]
+template <class T, int S> __device__ __forceinline__ void excl_products(const T *D, T *P) {
+  T pre[S];
+  pre[0] = T(1);
+#pragma unroll
+  for (int m = 1; m < S; m++) pre[m] = pre[m - 1] * D[m - 1];
+  T suf = T(1);
+#pragma unroll
+  for (int m = S - 1; m >= 0; m--) { P[m] = pre[m] * suf; suf *= D[m]; }
+}
+
+// FP64 weights with one division per normalisation: the reference's w_m = d_m/D_m followed by
+// w_m /= sum(w) equals d_m*P_m / sum_k d_k*P_k with P_m = prod_{j!=m} D_j (the sum always exceeds
+// the 1e-20 guard: the normalised weights sum to 1).  Likewise after map_rs, whose result is
+// A_m/B_m.  D >= 1e-20 and B >= ~1e-13 keep the products of S-1 factors far from underflow in
+// double.
+template <int ORD>
+__device__ __forceinline__ void weno_weights_fp64(const double *TV, bool do_map, double *wl, double *wr) {
+  using O = pu_weno::Ops<ORD>;
+  constexpr int S = O::S;
+  double D[S], P[S];
+#pragma unroll
+  for (int m = 0; m < S; m++) D[m] = TV[m] * TV[m] + 1.e-20;
+  excl_products<double, S>(D, P);
+  double sl = 0, sr = 0;
+#pragma unroll
+  for (int m = 0; m < S; m++) {
+    wl[m] = O::idealL(m) * P[m]; sl += wl[m];
+    wr[m] = O::idealR(m) * P[m]; sr += wr[m];
+  }
+  double rl = 1. / sl, rr = 1. / sr;
+#pragma unroll
+  for (int m = 0; m < S; m++) { wl[m] *= rl; wr[m] *= rr; }
+  if (!do_map) return;
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    double *w = side == 0 ? wl : wr;
+    double A[S], B[S], Q[S];
+#pragma unroll
+    for (int m = 0; m < S; m++) {
+      double d = side == 0 ? O::idealL(m) : O::idealR(m);
+      double x = w[m];
+      if constexpr (O::use_map_rs) {
+        double dl = x - d, d2 = dl * dl, t1 = d2 * d2 * d2, ww = x * (1. - x), t2 = ww * ww * ww;
+        B[m] = t1 + 2000. * t2;
+        A[m] = d * B[m] + t1 * dl;
+      } else {
+        B[m] = d * d + (1. - 2. * d) * x;
+        A[m] = x * (d + d * d - 3. * d * x + x * x);
+      }
+    }
+    excl_products<double, S>(B, Q);
+    double sum = 0;
+#pragma unroll
+    for (int m = 0; m < S; m++) { w[m] = A[m] * Q[m]; sum += w[m]; }
+    double r = 1. / sum;
+#pragma unroll
+    for (int m = 0; m < S; m++) w[m] *= r;
+  }
+}
+
 template <int ORD, class T>
 __device__ __forceinline__ void weno_edges(const T *s, bool do_map, bool imm_L, bool imm_R, T &qL, T &qR) {
   using O = pu_weno::Ops<ORD>;
@@ -150,6 +211,13 @@ __device__ __forceinline__ void weno_edges(const T *s, bool do_map, bool imm_L, 
     for (int m = 1; m < S; m++) mx = fmax(mx, TV[m]);
     if (imm_L) TV[S - 1] = mx;
     if (imm_R) TV[0] = mx;
+  }
+  if constexpr (std::is_same<T, double>::value) {
+    double wl[S], wr[S];
+    weno_weights_fp64<ORD>(TV, do_map, wl, wr);
+    qL = O::template combL<T>(s, wl);
+    qR = O::template combR<T>(s, wr);
+    return;
   }
   T rinv[S];
 #pragma unroll
@@ -221,9 +289,15 @@ __global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ prim, const
   int lo = 0, hi = ORD - 1;
   bool imm_L = false, imm_R = false, do_map = true, use_weno = true;
   if (do_recon) {
+    // A periodic neighbour cell past either end is the wrapped interior cell: its flags come from
+    // that cell's own position (the z halos of the flag array hold "immersed", not a periodic copy,
+    // dynamics_rk.h:1031-1034; the reference takes that face state from the wrapped cell's
+    // reconstruction through edge_exchange, :890-908).
+    const int cw = wrapi(c, nal);
+    const long long fbase = base + (long long)(cw - c) * sstride;
     uint8_t f[ORD];
 #pragma unroll
-    for (int s = 0; s < ORD; s++) f[s] = immf[base + (long long)(s - hs) * sstride];
+    for (int s = 0; s < ORD; s++) f[s] = immf[fbase + (long long)(s - hs) * sstride];
     imm_L = f[hs - 1];
     imm_R = f[hs + 1];
     if (!f[hs]) {
