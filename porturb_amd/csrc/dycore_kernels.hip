@@ -108,9 +108,18 @@ __global__ __launch_bounds__(256) void k_prep(const ST *__restrict__ U, const ST
 // ------------------------------------------------------------------------------------------------
 // WENO limiter (WenoLimiter.h compute_limited_edges / gll_1 / gll_2 / convexify / map_rs)
 // ------------------------------------------------------------------------------------------------
+// Reciprocal of a positive, normal argument: v_rcp_f32 in FP32; in FP64 v_rcp_f64 refined by two
+// Newton steps (about 1 ulp; every call site divides by a sum of positive terms >= 1e-20).
 template <class T> __device__ __forceinline__ T recip(T x) {
-  if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_rcpf(x);
-  else return T(1) / x;
+  if constexpr (std::is_same<T, float>::value) {
+    return __builtin_amdgcn_rcpf(x);
+  } else {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+  }
 }
 
 template <class T, int S> __device__ __forceinline__ void convexify(T *w) {
@@ -128,7 +137,7 @@ template <class T> __device__ __forceinline__ T map_rs(T w, T d) {     // k=6, m
   T dl = w - d;
   T d2 = dl * dl;
   T t1 = d2 * d2 * d2;
-  T ww = w * (T(1) - w);
+  T ww = fma(-w, w, w);
   T t2 = ww * ww * ww;
   return d + (t1 * dl) * recip(t1 + T(2000) * t2);
 }
@@ -148,41 +157,63 @@ template <class T, int S> __device__ __forceinline__ void excl_products(const T 
   for (int m = S - 1; m >= 0; m--) { P[m] = pre[m] * suf; suf *= D[m]; }
 }
 
-// FP64 weights with one division per normalisation: the reference's w_m = d_m/D_m followed by
-// w_m /= sum(w) equals d_m*P_m / sum_k d_k*P_k with P_m = prod_{j!=m} D_j (the sum always exceeds
-// the 1e-20 guard: the normalised weights sum to 1).  Likewise after map_rs, whose result is
-// A_m/B_m.  D >= 1e-20 and B >= ~1e-13 keep the products of S-1 factors far from underflow in
-// double.
+// FP64 limiter with few divisions and no redundant scaling (same mathematics as the reference,
+// WenoLimiter.h gll_1/gll_2, with a few roundings reordered):
+//  * TV normalisation: with tot = sum(TV) > 1e-20 the reference uses TV_m/tot, so
+//    d_m/((TV_m/tot)^2 + 1e-20) = tot^2 * d_m/(TV_m^2 + 1e-20*tot^2); the common factor tot^2
+//    cancels in the weight normalisation, leaving D_m = TV_m^2 + 1e-20*tot^2 (1e-20 when the guard
+//    fails).  imm_L/imm_R take a maximum, which commutes with the positive scaling.
+//  * w_m = d_m/D_m normalised equals d_m*P_m / sum_k d_k*P_k with P_m = prod_{j!=m} D_j
+//    (D >= 1e-20*tot^2 keeps products of S-1 factors far from underflow in double).
+//  * map_rs returns A_m/B_m; the renormalised mapped weights are A_m*Q_m / sum A_k*Q_k with
+//    Q_m = prod_{j!=m} B_j, and the final division is applied once to the face value.
 template <int ORD>
-__device__ __forceinline__ void weno_weights_fp64(const double *TV, bool do_map, double *wl, double *wr) {
+__device__ __forceinline__ void weno_edges_fp64(const double *s, bool do_map, bool imm_L, bool imm_R,
+                                                double &qL, double &qR) {
   using O = pu_weno::Ops<ORD>;
   constexpr int S = O::S;
+  double TV[S];
+  O::template tv<double>(s, TV);
+  double tot = TV[0];
+#pragma unroll
+  for (int m = 1; m < S; m++) tot += TV[m];
+  const double escale = tot > 1.e-20 ? 1.e-20 * (tot * tot) : 1.e-20;
+  if (imm_L || imm_R) {
+    double mx = TV[0];
+#pragma unroll
+    for (int m = 1; m < S; m++) mx = fmax(mx, TV[m]);
+    if (imm_L) TV[S - 1] = mx;
+    if (imm_R) TV[0] = mx;
+  }
   double D[S], P[S];
 #pragma unroll
-  for (int m = 0; m < S; m++) D[m] = TV[m] * TV[m] + 1.e-20;
+  for (int m = 0; m < S; m++) D[m] = fma(TV[m], TV[m], escale);
   excl_products<double, S>(D, P);
-  double sl = 0, sr = 0;
+  double wl[S], wr[S], sl = 0, sr = 0;
 #pragma unroll
   for (int m = 0; m < S; m++) {
     wl[m] = O::idealL(m) * P[m]; sl += wl[m];
     wr[m] = O::idealR(m) * P[m]; sr += wr[m];
   }
-  double rl = 1. / sl, rr = 1. / sr;
-#pragma unroll
-  for (int m = 0; m < S; m++) { wl[m] *= rl; wr[m] *= rr; }
-  if (!do_map) return;
+  if (!do_map) {
+    qL = O::template combL<double>(s, wl) * recip(sl);
+    qR = O::template combR<double>(s, wr) * recip(sr);
+    return;
+  }
+  const double rl = recip(sl), rr = recip(sr);
 #pragma unroll
   for (int side = 0; side < 2; side++) {
     double *w = side == 0 ? wl : wr;
+    const double r = side == 0 ? rl : rr;
     double A[S], B[S], Q[S];
 #pragma unroll
     for (int m = 0; m < S; m++) {
-      double d = side == 0 ? O::idealL(m) : O::idealR(m);
-      double x = w[m];
+      const double d = side == 0 ? O::idealL(m) : O::idealR(m);
+      const double x = w[m] * r;
       if constexpr (O::use_map_rs) {
-        double dl = x - d, d2 = dl * dl, t1 = d2 * d2 * d2, ww = x * (1. - x), t2 = ww * ww * ww;
-        B[m] = t1 + 2000. * t2;
-        A[m] = d * B[m] + t1 * dl;
+        double dl = x - d, d2 = dl * dl, t1 = d2 * d2 * d2, ww = fma(-x, x, x), t2 = ww * ww * ww;
+        B[m] = fma(2000., t2, t1);
+        A[m] = fma(d, B[m], t1 * dl);
       } else {
         B[m] = d * d + (1. - 2. * d) * x;
         A[m] = x * (d + d * d - 3. * d * x + x * x);
@@ -192,9 +223,8 @@ __device__ __forceinline__ void weno_weights_fp64(const double *TV, bool do_map,
     double sum = 0;
 #pragma unroll
     for (int m = 0; m < S; m++) { w[m] = A[m] * Q[m]; sum += w[m]; }
-    double r = 1. / sum;
-#pragma unroll
-    for (int m = 0; m < S; m++) w[m] *= r;
+    if (side == 0) qL = O::template combL<double>(s, w) * recip(sum);
+    else qR = O::template combR<double>(s, w) * recip(sum);
   }
 }
 
@@ -202,6 +232,10 @@ template <int ORD, class T>
 __device__ __forceinline__ void weno_edges(const T *s, bool do_map, bool imm_L, bool imm_R, T &qL, T &qR) {
   using O = pu_weno::Ops<ORD>;
   constexpr int S = O::S;
+  if constexpr (std::is_same<T, double>::value) {
+    weno_edges_fp64<ORD>(s, do_map, imm_L, imm_R, qL, qR);
+    return;
+  } else {
   T TV[S];
   O::template tv<T>(s, TV);
   convexify<T, S>(TV);
@@ -211,13 +245,6 @@ __device__ __forceinline__ void weno_edges(const T *s, bool do_map, bool imm_L, 
     for (int m = 1; m < S; m++) mx = fmax(mx, TV[m]);
     if (imm_L) TV[S - 1] = mx;
     if (imm_R) TV[0] = mx;
-  }
-  if constexpr (std::is_same<T, double>::value) {
-    double wl[S], wr[S];
-    weno_weights_fp64<ORD>(TV, do_map, wl, wr);
-    qL = O::template combL<T>(s, wl);
-    qR = O::template combR<T>(s, wr);
-    return;
   }
   T rinv[S];
 #pragma unroll
@@ -238,35 +265,59 @@ __device__ __forceinline__ void weno_edges(const T *s, bool do_map, bool imm_L, 
         wr[m] = map_w(wr[m], T(O::idealR(m)));
       }
     }
-    convexify<T, S>(wl);
-    convexify<T, S>(wr);
+    // renormalisation folded into the face value (the mapped weights sum to about one)
+    T sl = wl[0], sr = wr[0];
+#pragma unroll
+    for (int m = 1; m < S; m++) { sl += wl[m]; sr += wr[m]; }
+    qL = O::template combL<T>(s, wl) * recip(sl);
+    qR = O::template combR<T>(s, wr) * recip(sr);
+  } else {
+    qL = O::template combL<T>(s, wl);
+    qR = O::template combR<T>(s, wr);
   }
-  qL = O::template combL<T>(s, wl);
-  qR = O::template combR<T>(s, wr);
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
-// sweep along one direction: reconstruction + Riemann + divergence, face states through LDS
+// sweep along one direction: reconstruction + Riemann + divergence, face states through LDS.
+// A 256-thread block covers line segments of BS-2 cells along the sweep plus one neighbour cell at
+// each end (the cell whose reconstruction the segment's end faces need); x-contiguous threads keep
+// the loads of y and z sweeps coalesced.  Whole-line blocks (no neighbour cells) were measured
+// slower: 1024-thread blocks leave one block per CU, idle at every barrier.
 // ------------------------------------------------------------------------------------------------
-template <int DIR> struct SweepShape;
-template <> struct SweepShape<0> { static constexpr int BX = 64, BY = 4, BS = 64; };
-template <> struct SweepShape<1> { static constexpr int BX = 8, BY = 32, BS = 32; };
-template <> struct SweepShape<2> { static constexpr int BX = 8, BY = 32, BS = 32; };
+template <int DIR> struct SegShape;
+template <> struct SegShape<0> { static constexpr int BX = 64, BY = 4, BS = 64; };
+template <> struct SegShape<1> { static constexpr int BX = 8, BY = 32, BS = 32; };
+template <> struct SegShape<2> { static constexpr int BX = 8, BY = 32, BS = 32; };
+
+// acoustic upwind state of one face (dynamics_rk.h:640-651): mass flux, pressure, upwind side
+template <class T> struct Upw { T ru, p; bool left; };
+
+template <class T>
+__device__ __forceinline__ Upw<T> riemann(T rL, T uL, T pL, T rR, T uR, T pR, T hr) {
+  rL += hr; rR += hr;
+  const T cs = T(350.);
+  T ruL = uL * rL, ruR = uR * rR;
+  Upw<T> u;
+  u.p = T(0.5) * (pL + pR - cs * (ruR - ruL));
+  u.ru = T(0.5) * (ruL + ruR - (pR - pL) * T(1. / 350.));
+  u.left = u.ru > 0;
+  return u;
+}
 
 template <int ORD, class T, int DIR>
 __global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ prim, const uint8_t *__restrict__ immf,
                                                const uint8_t *__restrict__ any10, T *__restrict__ tend,
                                                SweepParams P) {
   constexpr int hs = (ORD - 1) / 2;
-  constexpr int BX = SweepShape<DIR>::BX, BY = SweepShape<DIR>::BY, BS = SweepShape<DIR>::BS;
+  constexpr int BX = SegShape<DIR>::BX, BY = SegShape<DIR>::BY, BS = SegShape<DIR>::BS;
   constexpr int NC = BS - 2;
   constexpr int NT = BX * BY;
-  __shared__ T bufA[3][NT];
-  __shared__ T bufB[2][NT];
+  __shared__ T bufA0[NT], bufA1[NT], bufA2[NT], bufB0[NT], bufB1[NT];
 
   const int tx = threadIdx.x, ty = threadIdx.y, lin = ty * BX + tx;
   const int t = DIR == 0 ? tx : ty;
-  const int nbs = DIR == 0 ? 1 : BX;
+  constexpr int nbs = DIR == 0 ? 1 : BX;
   int i, j, k, c, nal;
   if (DIR == 0) { c = blockIdx.x * NC - 1 + tx; i = c; j = blockIdx.y * BY + ty; k = blockIdx.z; nal = P.nx; }
   else if (DIR == 1) { i = blockIdx.x * BX + tx; c = blockIdx.y * NC - 1 + ty; j = c; k = blockIdx.z; nal = P.ny; }
@@ -307,7 +358,7 @@ __global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ prim, const
       for (int s = 0; s < hs; s++) if (f[s]) lo = s + 1;
     }
     int ci = i, cj = j, ck = k;
-    if (DIR == 0) ci = wrapi(c, nal); else if (DIR == 1) cj = wrapi(c, nal); else ck = wrapi(c, nal);
+    if (DIR == 0) ci = cw; else if (DIR == 1) cj = cw; else ck = cw;
     bool any = any10[((long long)ck * P.ny + cj) * P.nx + ci] != 0;
     do_map = !any;
     use_weno = any || P.weno_all;
@@ -354,31 +405,25 @@ __global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ prim, const
     recon(ln, false, qLu, qRu);
     recon(idP, true, qLp, qRp);
   }
-  bufA[0][lin] = qRr; bufA[1][lin] = qRu; bufA[2][lin] = qRp;
+  bufA0[lin] = qRr; bufA1[lin] = qRu; bufA2[lin] = qRp;
   __syncthreads();
-  T ru_upw = 0, p_upw = 0;
-  bool upL = true;
+  Upw<T> up{T(0), T(0), true};
   T Fr = 0, Fu = 0;
   if (has_face) {
-    T rL = bufA[0][lin - nbs], uL = bufA[1][lin - nbs], pL = bufA[2][lin - nbs];
+    T rL = bufA0[lin - nbs], uL = bufA1[lin - nbs], pL = bufA2[lin - nbs];
     T rR = qLr, uR = qLu, pR = qLp;
     if (bface_lo) { rL = rR; uL = uR; pL = pR; }
     if (bface_hi) { rR = rL; uR = uL; pR = pL; }
     if (wallz && (bface_lo || bface_hi)) { uL = 0; uR = 0; }     // W = 0 on both sides at walls
-    rL += hr; rR += hr;
-    const T cs = T(350.);
-    T ruL = uL * rL, ruR = uR * rR;
-    p_upw = T(0.5) * (pL + pR - cs * (ruR - ruL));
-    ru_upw = T(0.5) * (ruL + ruR - (pR - pL) / cs);
-    upL = ru_upw > 0;
-    Fr = ru_upw;
-    Fu = ru_upw * (upL ? uL : uR) + p_upw;
+    up = riemann(rL, uL, pL, rR, uR, pR, hr);
+    Fr = up.ru;
+    Fu = up.ru * (up.left ? uL : uR) + up.p;
   }
-  bufB[0][lin] = Fr; bufB[1][lin] = Fu;
+  bufB0[lin] = Fr; bufB1[lin] = Fu;
   __syncthreads();
   if (has_tend) {
-    accumulate(idR, -(bufB[0][lin + nbs] - bufB[0][lin]) * rd);
-    accumulate(ln, -(bufB[1][lin + nbs] - bufB[1][lin]) * rd);
+    accumulate(idR, -(bufB0[lin + nbs] - bufB0[lin]) * rd);
+    accumulate(ln, -(bufB1[lin + nbs] - bufB1[lin]) * rd);
   }
 
   // ---- phase B: the advected fields, one at a time ----
@@ -394,20 +439,20 @@ __global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ prim, const
     else { l = NUM_STATE + 1 + (b - 3); fidx = NUM_STATE + (b - 3); }
     T qL = 0, qR = 0;
     if (do_recon) recon(l, modify, qL, qR);
-    bufA[0][lin] = qR;
+    bufA0[lin] = qR;
     __syncthreads();
     T F = 0;
     if (has_face) {
-      T vL = bufA[0][lin - nbs], vR = qL;
+      T vL = bufA0[lin - nbs], vR = qL;
       if (bface_lo) vL = vR;
       if (bface_hi) vR = vL;
       if (wallz && (bface_lo || bface_hi) && l == idT) { vL = 0; vR = 0; }
       if (l == idT) { vL += ht; vR += ht; }
-      F = ru_upw * (upL ? vL : vR);
+      F = up.ru * (up.left ? vL : vR);
     }
-    bufB[0][lin] = F;
+    bufB0[lin] = F;
     __syncthreads();
-    if (has_tend) accumulate(fidx, -(bufB[0][lin + nbs] - bufB[0][lin]) * rd);
+    if (has_tend) accumulate(fidx, -(bufB0[lin + nbs] - bufB0[lin]) * rd);
   }
 }
 
@@ -583,14 +628,13 @@ static int sweep_one(pu_ctx *ctx, hipStream_t s) {
   P.rd = 1. / d;
   P.hy_rc = hy_cells(ctx, 0); P.hy_tc = hy_cells(ctx, 1);
   P.hy_re = hy_edges(ctx, 0); P.hy_te = hy_edges(ctx, 1);
-  constexpr int BX = SweepShape<DIR>::BX, BY = SweepShape<DIR>::BY, NC = SweepShape<DIR>::BS - 2;
+  static const char *names[3] = {"weno_x", "weno_y", "weno_z"};
+  LaunchTimer lt(ctx, names[DIR]);
+  constexpr int BX = SegShape<DIR>::BX, BY = SegShape<DIR>::BY, NC = SegShape<DIR>::BS - 2;
   dim3 blk(BX, BY, 1), grd;
   if (DIR == 0) grd = dim3((unsigned)((g.nx + NC - 1) / NC), (unsigned)((g.ny + BY - 1) / BY), (unsigned)g.nz);
   else if (DIR == 1) grd = dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)((g.ny + NC - 1) / NC), (unsigned)g.nz);
   else grd = dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)g.ny, (unsigned)((g.nz + NC - 1) / NC));
-  // A segment's last thread is the neighbour cell past the end; make sure the grid reaches cell n.
-  static const char *names[3] = {"weno_x", "weno_y", "weno_z"};
-  LaunchTimer lt(ctx, names[DIR]);
   hipLaunchKernelGGL((k_sweep<ORD, T, DIR>), grd, blk, 0, s, (const T *)ctx->d_prim, ctx->d_imm_flag,
                      ctx->d_any10, (T *)ctx->d_tend, P);
   PU_HIP_CHECK(hipGetLastError());

@@ -175,6 +175,40 @@ def emit(path):
         f.write("\n".join(out) + "\n")
 
 
+def tv_ldl(ord_):
+    """TV of each sub-stencil as a weighted sum of squares of first differences.
+
+    TV is translation invariant, so with d_j = s_{j+1} - s_j the coefficients a_n (n >= 1) of
+    sub-stencil m are a = M d (M[n][j] = sum_{i>j} C[n][i]) and TV = d^T G d with G = M^T Q M
+    (Q the quadratic form of coefs_to_tv).  The exact LDL^T factorisation G = L D L^T gives
+    TV = sum_k D_k (d_k + sum_{j>k} L[j][k] d_j)^2.  Returns per sub-stencil (D, L) as Fractions;
+    the differences are indexed relative to the sub-stencil's first point."""
+    t = tables(ord_)
+    N, S = t["N"], t["S"]
+    Q = [[F(0)] * N for _ in range(N)]
+    for (i, j), c in t["tv"].items():
+        if i == j:
+            Q[i][i] += c
+        else:
+            Q[i][j] += c / 2
+            Q[j][i] += c / 2
+    out = []
+    for m in range(S):
+        C = t["C"][m]
+        M = [[sum(C[n][i] for i in range(j + 1, N)) for j in range(N - 1)] for n in range(N)]
+        G = [[sum(M[a][p] * Q[a][b] * M[b][q] for a in range(1, N) for b in range(1, N))
+              for q in range(N - 1)] for p in range(N - 1)]
+        n = N - 1
+        L = [[F(int(i == j)) for j in range(n)] for i in range(n)]
+        D = [F(0)] * n
+        for j in range(n):
+            D[j] = G[j][j] - sum(L[j][k] ** 2 * D[k] for k in range(j))
+            for i in range(j + 1, n):
+                L[i][j] = (G[i][j] - sum(L[i][k] * L[j][k] * D[k] for k in range(j))) / D[j]
+        out.append((D, L))
+    return out
+
+
 def _lin_expr(coefs, var):
     """Straight-line sum of the nonzero terms c*var[i] (left to right)."""
     terms = ["T(%s)*%s[%d]" % (lit(c), var, i) for i, c in enumerate(coefs) if c != 0]
@@ -196,20 +230,22 @@ def emit_ops(path):
         out.append("template <> struct Ops<%d> {" % ord_)
         out.append("  static constexpr int ord = %d, hs = %d, N = %d, S = %d;" % (ord_, hs, N, S))
         out.append("  static constexpr bool use_map_rs = %s;" % ("true" if ord_ > 3 else "false"))
-        # TV of each sub-stencil
+        # TV of each sub-stencil: sum of D_k * (first-difference combination)^2 (see tv_ldl)
         out.append("  template <class T> __device__ static __forceinline__ void tv(const T *s, T *TV) {")
-        tv = sorted(t["tv"].items())
-        for m in range(S):
-            out.append("    {")
-            for n in range(1, N):
-                coefs = [t["C"][m][n][i] for i in range(N)]
-                terms = ["T(%s)*s[%d]" % (lit(c), m + i) for i, c in enumerate(coefs) if c != 0]
-                out.append("      const T a%d = %s;" % (n, " + ".join(terms)))
-            tvt = []
-            for (i, j), c in tv:
-                tvt.append("T(%s)*(a%d*a%d)" % (lit(c), i, j))
-            out.append("      TV[%d] = %s;" % (m, " + ".join(tvt)))
-            out.append("    }")
+        out.append("    T d[%d];" % (ord_ - 1))
+        out.append("    #pragma unroll")
+        out.append("    for (int i = 0; i < %d; i++) d[i] = s[i + 1] - s[i];" % (ord_ - 1))
+        for m, (Dk, L) in enumerate(tv_ldl(ord_)):
+            n = N - 1
+            terms = []
+            for k in range(n):
+                comb = "d[%d]" % (m + k)
+                for j in range(k + 1, n):
+                    if L[j][k] != 0:
+                        comb += " + T(%s)*d[%d]" % (lit(L[j][k]), m + j)
+                out.append("    const T c%d_%d = %s;" % (m, k, comb))
+                terms.append("T(%s)*(c%d_%d*c%d_%d)" % (lit(Dk[k]), m, k, m, k))
+            out.append("    TV[%d] = %s;" % (m, " + ".join(terms)))
         out.append("  }")
         for side in ("L", "R"):
             out.append("  template <class T> __device__ static __forceinline__ T comb%s(const T *s, const T *w) {" % side)

@@ -81,46 +81,47 @@ __device__ __forceinline__ void face_flux(const double *__restrict__ F, const Le
   auto S = [&](int f, long long idx) { return F[(long long)f * nh + idx]; };
   if (P.imm[m] == 1 || P.imm[c] == 1) { fc.zero = true; return; }
   fc.zero = false;
-  const double dx = P.dx, dy = P.dy, dz = P.dz;
+  const double rdx = 1. / P.dx, rdy = 1. / P.dy, rdz = 1. / P.dz;
+  const double r2dx = 0.5 * rdx, r2dy = 0.5 * rdy, r2dz = 0.5 * rdz;
   double du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz, dt_dz, dtn, dKn;
   double vmax;
   if (DIR == 0) {
-    du_dz = 0.5 * ((S(1, m + sz) - S(1, m - sz)) / (2 * dz) + (S(1, c + sz) - S(1, c - sz)) / (2 * dz));
-    dw_dz = 0.5 * ((S(3, m + sz) - S(3, m - sz)) / (2 * dz) + (S(3, c + sz) - S(3, c - sz)) / (2 * dz));
-    dt_dz = 0.5 * ((S(4, m + sz) - S(4, m - sz)) / (2 * dz) + (S(4, c + sz) - S(4, c - sz)) / (2 * dz));
-    du_dy = 0.5 * ((S(1, m + sy) - S(1, m - sy)) / (2 * dy) + (S(1, c + sy) - S(1, c - sy)) / (2 * dy));
-    dv_dy = 0.5 * ((S(2, m + sy) - S(2, m - sy)) / (2 * dy) + (S(2, c + sy) - S(2, c - sy)) / (2 * dy));
-    du_dx = (S(1, c) - S(1, m)) / dx;
-    dv_dx = (S(2, c) - S(2, m)) / dx;
-    dw_dx = (S(3, c) - S(3, m)) / dx;
-    dtn = (S(4, c) - S(4, m)) / dx;
-    dKn = (S(5, c) - S(5, m)) / dx;
+    du_dz = 0.5 * ((S(1, m + sz) - S(1, m - sz)) * r2dz + (S(1, c + sz) - S(1, c - sz)) * r2dz);
+    dw_dz = 0.5 * ((S(3, m + sz) - S(3, m - sz)) * r2dz + (S(3, c + sz) - S(3, c - sz)) * r2dz);
+    dt_dz = 0.5 * ((S(4, m + sz) - S(4, m - sz)) * r2dz + (S(4, c + sz) - S(4, c - sz)) * r2dz);
+    du_dy = 0.5 * ((S(1, m + sy) - S(1, m - sy)) * r2dy + (S(1, c + sy) - S(1, c - sy)) * r2dy);
+    dv_dy = 0.5 * ((S(2, m + sy) - S(2, m - sy)) * r2dy + (S(2, c + sy) - S(2, c - sy)) * r2dy);
+    du_dx = (S(1, c) - S(1, m)) * rdx;
+    dv_dx = (S(2, c) - S(2, m)) * rdx;
+    dw_dx = (S(3, c) - S(3, m)) * rdx;
+    dtn = (S(4, c) - S(4, m)) * rdx;
+    dKn = (S(5, c) - S(5, m)) * rdx;
     dv_dz = dw_dy = 0;
     vmax = P.visc_max_x;
   } else if (DIR == 1) {
-    dv_dz = 0.5 * ((S(2, m + sz) - S(2, m - sz)) / (2 * dz) + (S(2, c + sz) - S(2, c - sz)) / (2 * dz));
-    dw_dz = 0.5 * ((S(3, m + sz) - S(3, m - sz)) / (2 * dz) + (S(3, c + sz) - S(3, c - sz)) / (2 * dz));
-    dt_dz = 0.5 * ((S(4, m + sz) - S(4, m - sz)) / (2 * dz) + (S(4, c + sz) - S(4, c - sz)) / (2 * dz));
-    du_dx = 0.5 * ((S(1, m + sx) - S(1, m - sx)) / (2 * dx) + (S(1, c + sx) - S(1, c - sx)) / (2 * dx));
-    dv_dx = 0.5 * ((S(2, m + sx) - S(2, m - sx)) / (2 * dx) + (S(2, c + sx) - S(2, c - sx)) / (2 * dx));
-    du_dy = (S(1, c) - S(1, m)) / dy;
-    dv_dy = (S(2, c) - S(2, m)) / dy;
-    dw_dy = (S(3, c) - S(3, m)) / dy;
-    dtn = (S(4, c) - S(4, m)) / dy;
-    dKn = (S(5, c) - S(5, m)) / dy;
+    dv_dz = 0.5 * ((S(2, m + sz) - S(2, m - sz)) * r2dz + (S(2, c + sz) - S(2, c - sz)) * r2dz);
+    dw_dz = 0.5 * ((S(3, m + sz) - S(3, m - sz)) * r2dz + (S(3, c + sz) - S(3, c - sz)) * r2dz);
+    dt_dz = 0.5 * ((S(4, m + sz) - S(4, m - sz)) * r2dz + (S(4, c + sz) - S(4, c - sz)) * r2dz);
+    du_dx = 0.5 * ((S(1, m + sx) - S(1, m - sx)) * r2dx + (S(1, c + sx) - S(1, c - sx)) * r2dx);
+    dv_dx = 0.5 * ((S(2, m + sx) - S(2, m - sx)) * r2dx + (S(2, c + sx) - S(2, c - sx)) * r2dx);
+    du_dy = (S(1, c) - S(1, m)) * rdy;
+    dv_dy = (S(2, c) - S(2, m)) * rdy;
+    dw_dy = (S(3, c) - S(3, m)) * rdy;
+    dtn = (S(4, c) - S(4, m)) * rdy;
+    dKn = (S(5, c) - S(5, m)) * rdy;
     du_dz = dw_dx = 0;
     vmax = P.visc_max_y;
   } else {
-    du_dx = 0.5 * ((S(1, m + sx) - S(1, m - sx)) / (2 * dx) + (S(1, c + sx) - S(1, c - sx)) / (2 * dx));
-    dw_dx = 0.5 * ((S(3, m + sx) - S(3, m - sx)) / (2 * dx) + (S(3, c + sx) - S(3, c - sx)) / (2 * dx));
-    dv_dy = 0.5 * ((S(2, m + sy) - S(2, m - sy)) / (2 * dy) + (S(2, c + sy) - S(2, c - sy)) / (2 * dy));
-    dw_dy = 0.5 * ((S(3, m + sy) - S(3, m - sy)) / (2 * dy) + (S(3, c + sy) - S(3, c - sy)) / (2 * dy));
-    du_dz = (S(1, c) - S(1, m)) / dz;
-    dv_dz = (S(2, c) - S(2, m)) / dz;
-    dw_dz = (S(3, c) - S(3, m)) / dz;
-    dt_dz = (S(4, c) - S(4, m)) / dz;
+    du_dx = 0.5 * ((S(1, m + sx) - S(1, m - sx)) * r2dx + (S(1, c + sx) - S(1, c - sx)) * r2dx);
+    dw_dx = 0.5 * ((S(3, m + sx) - S(3, m - sx)) * r2dx + (S(3, c + sx) - S(3, c - sx)) * r2dx);
+    dv_dy = 0.5 * ((S(2, m + sy) - S(2, m - sy)) * r2dy + (S(2, c + sy) - S(2, c - sy)) * r2dy);
+    dw_dy = 0.5 * ((S(3, m + sy) - S(3, m - sy)) * r2dy + (S(3, c + sy) - S(3, c - sy)) * r2dy);
+    du_dz = (S(1, c) - S(1, m)) * rdz;
+    dv_dz = (S(2, c) - S(2, m)) * rdz;
+    dw_dz = (S(3, c) - S(3, m)) * rdz;
+    dt_dz = (S(4, c) - S(4, m)) * rdz;
     dtn = dt_dz;
-    dKn = (S(5, c) - S(5, m)) / dz;
+    dKn = (S(5, c) - S(5, m)) * rdz;
     du_dy = dv_dx = 0;
     vmax = P.visc_max_z;
   }
@@ -168,10 +169,11 @@ __global__ __launch_bounds__(256) void k_les_update(const double *__restrict__ F
   face_flux<1>(F, P, kh, jh + 1, ih, fy1);
   face_flux<2>(F, P, kh, jh, ih, fz0);
   face_flux<2>(F, P, kh + 1, jh, ih, fz1);
-  const double dx = P.dx, dy = P.dy, dz = P.dz;
+  const double rdx = 1. / P.dx, rdy = 1. / P.dy, rdz = 1. / P.dz;
+  const double r2dx = 0.5 * rdx, r2dy = 0.5 * rdy, r2dz = 0.5 * rdz;
   // TKE source (les_closure.h:233-278)
   double rho = S(0, c), K = S(5, c), t = S(4, c);
-  double dt_dz = (S(4, c + sz) - S(4, c - sz)) / (2 * dz);
+  double dt_dz = (S(4, c + sz) - S(4, c - sz)) * r2dz;
   double N = dt_dz >= 0 && P.enable_gravity ? sqrt(P.grav / t * dt_dz) : 0;
   double ell = fmin(0.76 * sqrt(K) / (N + 1.e-20), P.delta);
   double km = 0.1 * ell * sqrt(K);
@@ -180,19 +182,19 @@ __global__ __launch_bounds__(256) void k_les_update(const double *__restrict__ F
   const double *imm = P.imm;
   if (imm[c] < 1) {
     if (P.enable_gravity) src += -(P.grav * rho * km) / (t * Pr_t) * dt_dz;
-    src -= P.diss_mult * rho * (0.19 + 0.51 * ell / P.delta) / P.delta * pow(K, 1.5);
+    src -= P.diss_mult * rho * (0.19 + 0.51 * ell / P.delta) / P.delta * (K * sqrt(K));
     long long im1 = imm[c - sx] > 0 ? c : c - sx, ip1 = imm[c + sx] > 0 ? c : c + sx;
     long long jm1 = imm[c - sy] > 0 ? c : c - sy, jp1 = imm[c + sy] > 0 ? c : c + sy;
     long long km1 = imm[c - sz] > 0 ? c : c - sz, kp1 = imm[c + sz] > 0 ? c : c + sz;
-    double du_dx = (S(1, c + sx) - S(1, c - sx)) / (2 * dx);
-    double dv_dx = (S(2, ip1) - S(2, im1)) / (2 * dx);
-    double dw_dx = (S(3, ip1) - S(3, im1)) / (2 * dx);
-    double du_dy = (S(1, jp1) - S(1, jm1)) / (2 * dy);
-    double dv_dy = (S(2, c + sy) - S(2, c - sy)) / (2 * dy);
-    double dw_dy = (S(3, jp1) - S(3, jm1)) / (2 * dy);
-    double du_dz = (S(1, kp1) - S(1, km1)) / (2 * dz);
-    double dv_dz = (S(2, kp1) - S(2, km1)) / (2 * dz);
-    double dw_dz = (S(3, c + sz) - S(3, c - sz)) / (2 * dz);
+    double du_dx = (S(1, c + sx) - S(1, c - sx)) * r2dx;
+    double dv_dx = (S(2, ip1) - S(2, im1)) * r2dx;
+    double dw_dx = (S(3, ip1) - S(3, im1)) * r2dx;
+    double du_dy = (S(1, jp1) - S(1, jm1)) * r2dy;
+    double dv_dy = (S(2, c + sy) - S(2, c - sy)) * r2dy;
+    double dw_dy = (S(3, jp1) - S(3, jm1)) * r2dy;
+    double du_dz = (S(1, kp1) - S(1, km1)) * r2dz;
+    double dv_dz = (S(2, kp1) - S(2, km1)) * r2dz;
+    double dw_dz = (S(3, c + sz) - S(3, c - sz)) * r2dz;
     double j1_i1 = (du_dx + du_dx) * du_dx, j1_i2 = (dv_dx + du_dy) * dv_dx, j1_i3 = (dw_dx + du_dz) * dw_dx;
     double j2_i1 = (du_dy + dv_dx) * du_dy, j2_i2 = (dv_dy + dv_dy) * dv_dy, j2_i3 = (dw_dy + dv_dz) * dw_dy;
     double j3_i1 = (du_dz + dw_dx) * du_dz, j3_i2 = (dv_dz + dw_dy) * dv_dz, j3_i3 = (dw_dz + dw_dz) * dw_dz;
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(256) void k_les_update(const double *__restrict__ F
   auto fv = [](const FaceCommon &fc, int f) { return fc.zero ? 0. : fc.f[f]; };
   double tend[5];
   for (int f = 0; f < 5; f++)
-    tend[f] = -(fv(fx1, f) - fv(fx0, f)) / dx - (fv(fy1, f) - fv(fy0, f)) / dy - (fv(fz1, f) - fv(fz0, f)) / dz;
+    tend[f] = -(fv(fx1, f) - fv(fx0, f)) * rdx - (fv(fy1, f) - fv(fy0, f)) * rdy - (fv(fz1, f) - fv(fz0, f)) * rdz;
   tend[4] += src;
   double dt = P.dt;
   double ru = S(1, c) * rho + dt * tend[0];
@@ -217,13 +219,13 @@ __global__ __launch_bounds__(256) void k_les_update(const double *__restrict__ F
   // diffused tracers: flux -rho*visc*dq/dn on each open face
   for (int q = 0; q < P.nq; q++) {
     const double *Q = F + (long long)(6 + q) * nh;
-    double qx0 = fx0.zero ? 0. : -fx0.rho * fx0.visc * ((Q[c] - Q[c - sx]) / dx);
-    double qx1 = fx1.zero ? 0. : -fx1.rho * fx1.visc * ((Q[c + sx] - Q[c]) / dx);
-    double qy0 = fy0.zero ? 0. : -fy0.rho * fy0.visc * ((Q[c] - Q[c - sy]) / dy);
-    double qy1 = fy1.zero ? 0. : -fy1.rho * fy1.visc * ((Q[c + sy] - Q[c]) / dy);
-    double qz0 = fz0.zero ? 0. : -fz0.rho * fz0.visc * ((Q[c] - Q[c - sz]) / dz);
-    double qz1 = fz1.zero ? 0. : -fz1.rho * fz1.visc * ((Q[c + sz] - Q[c]) / dz);
-    double tq = -(qx1 - qx0) / dx - (qy1 - qy0) / dy - (qz1 - qz0) / dz;
+    double qx0 = fx0.zero ? 0. : -fx0.rho * fx0.visc * ((Q[c] - Q[c - sx]) * rdx);
+    double qx1 = fx1.zero ? 0. : -fx1.rho * fx1.visc * ((Q[c + sx] - Q[c]) * rdx);
+    double qy0 = fy0.zero ? 0. : -fy0.rho * fy0.visc * ((Q[c] - Q[c - sy]) * rdy);
+    double qy1 = fy1.zero ? 0. : -fy1.rho * fy1.visc * ((Q[c + sy] - Q[c]) * rdy);
+    double qz0 = fz0.zero ? 0. : -fz0.rho * fz0.visc * ((Q[c] - Q[c - sz]) * rdz);
+    double qz1 = fz1.zero ? 0. : -fz1.rho * fz1.visc * ((Q[c + sz] - Q[c]) * rdz);
+    double tq = -(qx1 - qx0) * rdx - (qy1 - qy0) * rdy - (qz1 - qz0) * rdz;
     trc.p[P.qidx[q]][id] = Q[c] * rho + dt * tq;
   }
 }

@@ -80,3 +80,33 @@ def test_ideal_weights_are_textbook():
     """WENO-5 0.3/0.6/0.1 and WENO-9 (1, 20, 60, 40, 5)/126 (Balsara & Shu 2000)."""
     np.testing.assert_allclose(golden_for(5)["idealL"], [0.3, 0.6, 0.1], rtol=1e-15)
     np.testing.assert_allclose(golden_for(9)["idealR"], np.array([1, 20, 60, 40, 5]) / 126., rtol=1e-15)
+
+
+@pytest.mark.parametrize("ord_", ORDERS)
+def test_difference_form_of_tv_is_exact(ord_):
+    """The kernels evaluate TV as sum_k D_k (d_k + sum_j L_jk d_j)^2 over first differences d; in
+    exact rational arithmetic that equals the reference's coefs_to_tv(coefs{N}_shift{m}(s)) for
+    every sub-stencil (reference constants from the golden file, stencils random rationals)."""
+    from fractions import Fraction as Fr
+    import random
+    g = golden_for(ord_)
+    N = (ord_ - 1) // 2 + 1
+    t = gen.tables(ord_)
+    ldl = gen.tv_ldl(ord_)
+    rnd = random.Random(ord_)
+    for trial in range(5):
+        s = [Fr(rnd.randint(-1000, 1000), rnd.randint(1, 50)) for _ in range(ord_)]
+        d = [s[i + 1] - s[i] for i in range(ord_ - 1)]
+        for m in range(N):
+            a = [sum(t["C"][m][n][i] * s[m + i] for i in range(N)) for n in range(N)]
+            tv_ref = sum(c * a[i] * a[j] for (i, j), c in t["tv"].items())
+            D, L = ldl[m]
+            tv_ldl = Fr(0)
+            for k in range(N - 1):
+                comb = d[m + k] + sum(L[j][k] * d[m + j] for j in range(k + 1, N - 1))
+                tv_ldl += D[k] * comb * comb
+            assert tv_ldl == tv_ref
+            # and the rational table agrees with the reference's decimal constants
+            a_f = [float(x) for x in a]
+            tv_gold = sum(g["tv"][i, j] * a_f[i] * a_f[j] for i in range(N) for j in range(i, N))
+            assert float(tv_ref) == pytest.approx(tv_gold, rel=1e-12, abs=1e-9)

@@ -38,14 +38,20 @@ CASES = [
     dict(ord=5, bc_z=pa.PU_BC_PERIODIC),
     dict(ord=9, weno_all=0),
     dict(ord=5, latitude=40.),
+    # whole-line thread blocks: x lines (nx a multiple of 64) and z columns (nx a multiple of 8)
+    dict(ord=9, nx=64, ny=12, nz=16),
+    dict(ord=5, nx=64, ny=12, nz=16, bc_x=pa.PU_BC_PRECURSOR),
+    dict(ord=5, nx=64, ny=12, nz=16, bc_z=pa.PU_BC_PERIODIC),
+    dict(ord=9, nx=24, ny=20, nz=16, bc_z=pa.PU_BC_PERIODIC),
+    dict(ord=11, nx=24, ny=20, nz=20),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[str(c) for c in CASES])
 def test_tendencies_match_oracle(case):
     kw = dict(case)
-    nz = kw.pop("nz", 16)
-    cfg = small_config(nx=22, ny=20, nz=nz, tracers=[WV, TKE, P1], **kw)
+    nz, nx, ny = kw.pop("nz", 16), kw.pop("nx", 22), kw.pop("ny", 20)
+    cfg = small_config(nx=nx, ny=ny, nz=nz, tracers=[WV, TKE, P1], **kw)
     st, imm = random_state(cfg, seed=21, buildings=3)
     print("case", case, flush=True)
     d, cs = gpu_setup(cfg, st, imm)
