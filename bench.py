@@ -88,16 +88,12 @@ def main():
     n3 = cfg["nx"] * cfg["ny"] * cfg["nz"]
     print("bench: %s, %d cells, warmup %d, steps %d" % (cfg["name"], n3, args.warmup, args.steps),
           file=sys.stderr, flush=True)
-    cs = pa.CouplerState(cfg)
-    pa.case_init(cfg, cfg["case_id"], 1234, cs)
-    d = pa.Dycore(cfg)
-    d.init(cs)
-    dt = d.compute_time_step()
+    # the drop-in C++ path: core::Coupler + Dynamics_Euler_Stratified_WenoFV (+ LES_Closure)
+    d = pa.Driver(cfg["case_id"], cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlen"] / cfg["nx"], ord=cfg["ord"],
+                  precision=cfg["precision"], seed=1234, closure=les)
 
     def step():
-        d.time_step(cs, dt)
-        if les:
-            d.les_apply(cs, dt)
+        d.step(1)
 
     for _ in range(args.warmup):
         step()
@@ -129,21 +125,30 @@ def main():
     peak = rl.MI355X["fp64_TFLOPs"] if fp64 else rl.MI355X["fp32_TFLOPs"]
     step_bytes = rl.step_bytes_per_cell(5 + cfg["num_tracers"], fp64)
     hbm_frac = value * step_bytes / (rl.MI355X["hbm_GBps"] * 1e9)
+    # HBM bytes per launch of the dominant kernel, from the calibrated FETCH_SIZE/WRITE_SIZE runs
+    # recorded under profiles/ for this workload (null for workloads not profiled)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s.json" % cfg["name"].replace("neutral_urban_abl", "abl_urban")
+                         .replace("_weno9_fp64", "_fp64"))
+    if os.path.exists(tfile):
+        tj = json.load(open(tfile))
+        key = "k_sweep<%d, %s, %d>" % (cfg["ord"], "double" if fp64 else "float", "xyz".index(dom[-1]))
+        if key in tj["bytes_per_launch"]:
+            traffic = tj["bytes_per_launch"][key]["read"] + tj["bytes_per_launch"][key]["write"]
     roof = {"bound": "valu", "kernel": "k_sweep<%d,%s,%s>" % (cfg["ord"], "double" if fp64 else "float", dom[-1]),
             "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": None,
+            "frac": round(achieved / peak, 4), "traffic": traffic,
             "flops_per_launch": flops_launch, "avg_launch_ms": round(avg_ms, 4),
             "step_hbm_frac": round(hbm_frac, 4), "step_bytes_per_cell": step_bytes,
             "kernel_ms_per_step": {f: round(times[f][0] / 2, 3) for f in fams}}
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64" if fp64 else "f32", "data": "synthetic",
-           "config": {"workload": cfg["name"], "cells": n3, "ord": cfg["ord"],
+           "config": {"workload": cfg["name"], "cells": n3, "ord": cfg["ord"], "path": "C++ Coupler + modules",
                       "closure": "tke" if les else "none", "tracers": cfg["num_tracers"],
                       "step": "dycore.time_step + les_closure.apply" if les else "dycore.time_step"},
            "roofline": roof}
     d.destroy()
-    cs = None
     if not args.no_cpu_baseline:
         print("bench: CPU oracle baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cfg, les, args.cpu_seconds)

@@ -105,6 +105,11 @@ int  pu_dycore_hydrostatics(pu_ctx *ctx, double *dens_c, double *theta_c, double
 /* dycore_any_immersed10 and dycore_immersed_tau (host outputs, [nz][ny][nx]) (:389-425,:1127-1143). */
 int  pu_dycore_immersed_masks(pu_ctx *ctx, unsigned char *any10, int *tau);
 
+/* The immersed_proportion_halos entry the LES closure reads (built by custom_modules::sc_init,
+ * sc_init.h:728-767: halo 1, x/y periodic, bottom halo 1, top halo 0).  pu_dycore_init builds it
+ * too; this entry point serves contexts that run the closure without the dycore. */
+int  pu_les_closure_init(pu_ctx *ctx, const pu_coupler_state *cs);
+
 /* LES_Closure::apply (les_closure.h:30-323).  In place on the coupler state; requires idTKE. */
 int  pu_les_closure_apply(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys);
 
@@ -119,7 +124,9 @@ int  pu_alloc_device(size_t bytes, void **ptr);
 int  pu_free_device(void *ptr);
 int  pu_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int  pu_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int  pu_memcpy_d2d(void *dst, const void *src, size_t bytes);   /* device deep_copy_to; a plain copy kernel */
 int  pu_memset_device(void *ptr, int value, size_t bytes);
+int  pu_fill_device(double *ptr, double value, size_t count);   /* YAKL scalar assignment (arr = v) */
 int  pu_synchronize(pu_ctx *ctx);            /* Kokkos::fence() on the context's stream */
 
 /* Seeded synthetic initial state (the role of custom_modules::sc_init + sc_perturb for the
@@ -139,6 +146,21 @@ int  pu_timer_start(pu_ctx *ctx);
 int  pu_timer_stop(pu_ctx *ctx, double *ms);
 int  pu_kernel_timing(pu_ctx *ctx, int enable);
 int  pu_kernel_time(pu_ctx *ctx, const char *name, double *total_ms, long long *count);
+
+/* Benchmark driver on the drop-in C++ API (core::Coupler + modules), the analog of the reference's
+ * experiments/examples/abl_neutral.cpp:65-179 main loop: constants as sc_init.h:31-51, tracers
+ * water_vapor (+ TKE from LES_Closure::init when closure != 0, + passive_1/2 for the ABL cases),
+ * pu_case_init, dycore init; pu_driver_step runs run_module(dycore.time_step) and
+ * run_module(les_closure.apply) with dt = compute_time_step, nsteps times. */
+typedef struct pu_driver pu_driver;
+int  pu_driver_create(int case_id, int nx, int ny, int nz, double dx, int ord, int precision,
+                      unsigned long long seed, int closure, pu_driver **out);
+int  pu_driver_step(pu_driver *d, int nsteps);
+int  pu_driver_state(pu_driver *d, pu_coupler_state *out, int *num_tracers);
+pu_ctx *pu_driver_context(pu_driver *d);
+int  pu_driver_tracer_name(pu_driver *d, int index, char *buf, int buflen);
+const char *pu_driver_last_error(void);
+int  pu_driver_destroy(pu_driver *d);
 
 /* Library build information (gfx target, precision variants compiled). */
 const char *pu_build_info(void);

@@ -87,6 +87,16 @@ def lib():
         "pu_timer_stop": (I, [P, C.POINTER(D)]),
         "pu_kernel_timing": (I, [P, I]),
         "pu_kernel_time": (I, [P, C.c_char_p, C.POINTER(D), C.POINTER(C.c_longlong)]),
+        "pu_les_closure_init": (I, [P, csp]),
+        "pu_fill_device": (I, [P, D, S]),
+        "pu_memcpy_d2d": (I, [P, P, S]),
+        "pu_driver_create": (I, [I, I, I, I, D, I, I, C.c_ulonglong, I, C.POINTER(P)]),
+        "pu_driver_step": (I, [P, I]),
+        "pu_driver_state": (I, [P, csp, C.POINTER(I)]),
+        "pu_driver_context": (P, [P]),
+        "pu_driver_tracer_name": (I, [P, I, C.c_char_p, I]),
+        "pu_driver_last_error": (C.c_char_p, []),
+        "pu_driver_destroy": (I, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -306,3 +316,78 @@ class Dycore:
 def case_init(cfg, case_id, seed, cs):
     pc = to_pu_config(cfg)
     check(lib().pu_case_init(C.byref(pc), case_id, seed, C.byref(cs.c), cs.immersed.ptr))
+
+
+class Driver:
+    """The C++ benchmark driver (core::Coupler + Dynamics_Euler_Stratified_WenoFV + LES_Closure)
+    for one case; the coupler's DataManager owns the state."""
+
+    def __init__(self, case_id, nx, ny, nz, dx, ord=9, precision=PU_FP64, seed=1234, closure=True):
+        p = C.c_void_p()
+        rc = lib().pu_driver_create(case_id, nx, ny, nz, dx, ord, precision, seed, int(closure), C.byref(p))
+        if rc != PU_OK:
+            raise RuntimeError("pu_driver_create: %s %s" % (lib().pu_driver_last_error().decode(),
+                                                          lib().pu_last_error().decode()))
+        self.d = p.value
+        self.shape = (nz, ny, nx)
+        self.ctx = lib().pu_driver_context(self.d)
+
+    def step(self, n=1):
+        if lib().pu_driver_step(self.d, n) != PU_OK:
+            raise RuntimeError("pu_driver_step: " + lib().pu_driver_last_error().decode())
+
+    def tracer_names(self):
+        cs, n = PuCouplerState(), C.c_int()
+        check(lib().pu_driver_state(self.d, C.byref(cs), C.byref(n)))
+        out = []
+        for t in range(n.value):
+            buf = C.create_string_buffer(64)
+            check(lib().pu_driver_tracer_name(self.d, t, buf, 64))
+            out.append(buf.value.decode())
+        return out
+
+    def download(self):
+        """{name: array} of density_dry, uvel, vvel, wvel, temp and every tracer."""
+        cs, n = PuCouplerState(), C.c_int()
+        check(lib().pu_driver_state(self.d, C.byref(cs), C.byref(n)))
+        out = {}
+        nb = int(np.prod(self.shape)) * 8
+        for f in CouplerState.FIELDS:
+            a = np.empty(self.shape)
+            check(lib().pu_memcpy_d2h(a.ctypes.data, getattr(cs, f), nb))
+            out[f] = a
+        for t, name in enumerate(self.tracer_names()):
+            a = np.empty(self.shape)
+            check(lib().pu_memcpy_d2h(a.ctypes.data, cs.tracers[t], nb))
+            out[name] = a
+        return out
+
+    def synchronize(self):
+        check(lib().pu_synchronize(self.ctx))
+
+    def timer_start(self):
+        check(lib().pu_timer_start(self.ctx))
+
+    def timer_stop(self):
+        ms = C.c_double()
+        check(lib().pu_timer_stop(self.ctx, C.byref(ms)))
+        return ms.value
+
+    def kernel_timing(self, on):
+        check(lib().pu_kernel_timing(self.ctx, int(on)))
+
+    def kernel_time(self, name):
+        ms, n = C.c_double(), C.c_longlong()
+        check(lib().pu_kernel_time(self.ctx, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def destroy(self):
+        if self.d:
+            lib().pu_driver_destroy(self.d)
+            self.d = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -221,10 +221,20 @@ int pu_dycore_immersed_masks(pu_ctx *ctx, unsigned char *any10, int *tau) {
   return PU_OK;
 }
 
+int pu_les_closure_init(pu_ctx *ctx, const pu_coupler_state *cs) {
+  int rc = check_state(ctx, cs, true);
+  if (rc) return rc;
+  rc = les_init_device(ctx, cs);
+  if (rc) return rc;
+  ctx->les_ready = true;
+  return PU_OK;
+}
+
 int pu_les_closure_apply(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys) {
   int rc = check_state(ctx, cs, false);
   if (rc) return rc;
-  if (!ctx->dycore_ready) return fail(PU_ERR_STATE, "pu_dycore_init has not been called (immersed halos)");
+  if (!ctx->dycore_ready && !ctx->les_ready)
+    return fail(PU_ERR_STATE, "pu_les_closure_init or pu_dycore_init has not been called (immersed halos)");
   if (!(dt_phys > 0)) return fail(PU_ERR_ARG, "dt_phys must be positive");
   rc = les_apply_device(ctx, cs, dt_phys);
   if (rc) return rc;
@@ -246,6 +256,14 @@ int pu_free_device(void *ptr) { PU_HIP_CHECK(hipFree(ptr)); return PU_OK; }
 int pu_memcpy_h2d(void *dst, const void *src, size_t bytes) { PU_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); return PU_OK; }
 int pu_memcpy_d2h(void *dst, const void *src, size_t bytes) { PU_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); return PU_OK; }
 int pu_memset_device(void *ptr, int value, size_t bytes) { PU_HIP_CHECK(hipMemset(ptr, value, bytes)); return PU_OK; }
+int pu_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  if ((!dst || !src) && bytes) return fail(PU_ERR_ARG, "null pointer");
+  return copy_device(dst, src, bytes);
+}
+int pu_fill_device(double *ptr, double value, size_t count) {
+  if (!ptr && count) return fail(PU_ERR_ARG, "null pointer");
+  return fill_device(ptr, value, count);
+}
 int pu_synchronize(pu_ctx *ctx) { PU_HIP_CHECK(hipStreamSynchronize(ctx ? ctx->stream : nullptr)); return PU_OK; }
 
 int pu_case_init(const pu_config *cfg, int case_id, unsigned long long seed, pu_coupler_state *cs, double *imm) {

@@ -1,0 +1,121 @@
+// driver.cpp -- benchmark driver on the drop-in C++ API (core::Coupler + modules).
+//
+// Mirrors the structure of the reference's ABL drivers (experiments/examples/abl_neutral.cpp:
+// 65-179): set options and constants as custom_modules::sc_init does (sc_init.h:31-51), allocate
+// the coupled state, add tracers, initialise the case, init the LES closure and the dycore, then
+// per step run_module(dycore.time_step) and run_module(les_closure.apply) with
+// dt = dycore.compute_time_step.  Exposed through pu_driver_* for bench.py and the tests.
+#include <cmath>
+#include <memory>
+#include <string>
+
+#include "porturb.h"
+#include "porturb/core/coupler.h"
+#include "porturb/modules/dynamics_rk.h"
+#include "porturb/modules/les_closure.h"
+
+struct pu_driver {
+  core::Coupler coupler;
+  modules::Dynamics_Euler_Stratified_WenoFV dycore;
+  modules::LES_Closure les;
+  bool closure = true;
+  std::string error;
+};
+
+static thread_local std::string g_driver_error;
+
+extern "C" {
+
+int pu_driver_create(int case_id, int nx, int ny, int nz, double dx, int ord, int precision,
+                     unsigned long long seed, int closure, pu_driver **out) {
+  if (!out) return PU_ERR_ARG;
+  *out = nullptr;
+  try {
+    auto d = std::make_unique<pu_driver>();
+    core::Coupler &c = d->coupler;
+    d->closure = closure != 0;
+    // constants (sc_init.h:31-51)
+    double R_d = 287., cp_d = 1003., R_v = 461., cp_v = 1859., p0 = 1.e5, grav = 9.81;
+    double cv_d = cp_d - R_d, gamma = cp_d / cv_d, kappa = R_d / cp_d;
+    c.set_option<double>("R_d", R_d); c.set_option<double>("cp_d", cp_d); c.set_option<double>("R_v", R_v);
+    c.set_option<double>("cp_v", cp_v); c.set_option<double>("p0", p0); c.set_option<double>("grav", grav);
+    c.set_option<double>("cv_d", cv_d); c.set_option<double>("gamma_d", gamma); c.set_option<double>("kappa_d", kappa);
+    c.set_option<double>("C0", std::pow(R_d * std::pow(p0, -kappa), gamma));
+    c.set_option<std::string>("bc_x", case_id == PU_CASE_WINDFARM ? "precursor" : "periodic");
+    c.set_option<std::string>("bc_y", "periodic");
+    c.set_option<std::string>("bc_z", "solid_wall");
+    c.set_option<int>("porturb_ord", ord);
+    c.set_option<std::string>("dycore_precision", precision == PU_FP32 ? "fp32" : "fp64");
+    c.set_option<bool>("function_timers", false);
+    c.distribute_mpi_and_allocate_coupled_state(nz, (size_t)ny, (size_t)nx);
+    c.set_grid(nx * dx, ny * dx, nz * dx);
+    auto &dm = c.get_data_manager_readwrite();
+    for (const char *name : {"density_dry", "uvel", "vvel", "wvel", "temp", "immersed_proportion"})
+      dm.register_and_allocate<double>(name, "", {nz, ny, nx}, {"z", "y", "x"});
+    c.add_tracer("water_vapor", "water_vapor", true, true, true);
+    c.set_option<int>("idWV", c.get_tracer_index("water_vapor"));
+    if (d->closure) d->les.init(c);                  // adds TKE (= 0.1)
+    if (case_id != PU_CASE_BUBBLE) {
+      c.add_tracer("passive_1", "passive tracer", true, false, true);
+      c.add_tracer("passive_2", "passive tracer", true, false, true);
+    }
+    // initial state on the device (the role of sc_init + sc_perturb)
+    pu_config cfg = modules::detail::make_config(c, ord);
+    pu_coupler_state cs = c.coupler_state();
+    double *imm = dm.get<double, 3>("immersed_proportion").data();
+    if (pu_case_init(&cfg, case_id, seed, &cs, imm) != PU_OK) core::endrun(pu_last_error());
+    d->dycore.init(c);
+    *out = d.release();
+    return PU_OK;
+  } catch (std::exception const &e) {
+    g_driver_error = e.what();
+    return PU_ERR_ARG;
+  }
+}
+
+int pu_driver_step(pu_driver *d, int nsteps) {
+  if (!d) return PU_ERR_ARG;
+  try {
+    core::Coupler &c = d->coupler;
+    for (int n = 0; n < nsteps; n++) {
+      double dt = d->dycore.compute_time_step(c);
+      c.run_module([&](core::Coupler &cc) { d->dycore.time_step(cc, dt); }, "dycore");
+      if (d->closure) c.run_module([&](core::Coupler &cc) { d->les.apply(cc, dt); }, "les_closure");
+      c.set_option<double>("elapsed_time", c.get_option<double>("elapsed_time") + dt);
+    }
+    return PU_OK;
+  } catch (std::exception const &e) {
+    g_driver_error = e.what();
+    return PU_ERR_STATE;
+  }
+}
+
+int pu_driver_state(pu_driver *d, pu_coupler_state *out, int *num_tracers) {
+  if (!d || !out) return PU_ERR_ARG;
+  *out = d->coupler.coupler_state();
+  if (num_tracers) *num_tracers = d->coupler.get_num_tracers();
+  return PU_OK;
+}
+
+pu_ctx *pu_driver_context(pu_driver *d) {
+  if (!d) return nullptr;
+  auto &slot = d->coupler.module_state("porturb_context");
+  return slot ? static_cast<modules::detail::Context *>(slot.get())->ctx : nullptr;
+}
+
+int pu_driver_tracer_name(pu_driver *d, int index, char *buf, int buflen) {
+  if (!d || !buf || buflen < 1) return PU_ERR_ARG;
+  auto names = d->coupler.get_tracer_names();
+  if (index < 0 || index >= (int)names.size()) return PU_ERR_ARG;
+  snprintf(buf, buflen, "%s", names[index].c_str());
+  return PU_OK;
+}
+
+const char *pu_driver_last_error(void) { return g_driver_error.c_str(); }
+
+int pu_driver_destroy(pu_driver *d) {
+  delete d;
+  return PU_OK;
+}
+
+}  // extern "C"

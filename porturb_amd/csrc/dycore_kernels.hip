@@ -318,10 +318,12 @@ __global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ prim, const
   const int tx = threadIdx.x, ty = threadIdx.y, lin = ty * BX + tx;
   const int t = DIR == 0 ? tx : ty;
   constexpr int nbs = DIR == 0 ? 1 : BX;
+  unsigned bx, by, bz;
+  swizzled_block(bx, by, bz);
   int i, j, k, c, nal;
-  if (DIR == 0) { c = blockIdx.x * NC - 1 + tx; i = c; j = blockIdx.y * BY + ty; k = blockIdx.z; nal = P.nx; }
-  else if (DIR == 1) { i = blockIdx.x * BX + tx; c = blockIdx.y * NC - 1 + ty; j = c; k = blockIdx.z; nal = P.ny; }
-  else { i = blockIdx.x * BX + tx; j = blockIdx.y; c = blockIdx.z * NC - 1 + ty; k = c; nal = P.nz; }
+  if (DIR == 0) { c = bx * NC - 1 + tx; i = c; j = by * BY + ty; k = bz; nal = P.nx; }
+  else if (DIR == 1) { i = bx * BX + tx; c = by * NC - 1 + ty; j = c; k = bz; nal = P.ny; }
+  else { i = bx * BX + tx; j = by; c = bz * NC - 1 + ty; k = c; nal = P.nz; }
   const bool across_ok = DIR == 0 ? (j < P.ny) : (i < P.nx);
   const bool in_range = across_ok && c >= -1 && c <= nal;
   const bool periodic = P.bc_along == PU_BC_PERIODIC;

@@ -148,6 +148,51 @@ __global__ __launch_bounds__(256) void k_imm_les(const double *prop, double *out
   out[id] = v;
 }
 
+int les_init_device(pu_ctx *ctx, const pu_coupler_state *cs) {
+  const pu::Grid &g = ctx->g;
+  long long nl = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+  hipLaunchKernelGGL(k_imm_les, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, ctx->stream, cs->immersed_proportion,
+                     ctx->d_imm_les_val, g.nx, g.ny, g.nz);
+  PU_HIP_CHECK(hipGetLastError());
+  PU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return PU_OK;
+}
+
+__global__ __launch_bounds__(256) void k_fill(double *p, double v, long long n) {
+  long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id < n) p[id] = v;
+}
+
+// plain streaming copy (16-byte vectors, grid-stride); also the known-size kernel used to
+// calibrate the HBM traffic counters
+__global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long long n16) {
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < n16; id += (long long)gridDim.x * blockDim.x)
+    dst[id] = src[id];
+}
+
+int copy_device(void *dst, const void *src, size_t bytes) {
+  if (!bytes) return PU_OK;
+  if ((bytes % 16) || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16)) {
+    PU_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return PU_OK;
+  }
+  long long n16 = (long long)(bytes / 16);
+  long long blocks = (n16 + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, nullptr, (const uint4 *)src, (uint4 *)dst, n16);
+  PU_HIP_CHECK(hipGetLastError());
+  PU_HIP_CHECK(hipDeviceSynchronize());
+  return PU_OK;
+}
+
+int fill_device(double *ptr, double value, size_t count) {
+  if (!count) return PU_OK;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, nullptr, ptr, value, (long long)count);
+  PU_HIP_CHECK(hipGetLastError());
+  PU_HIP_CHECK(hipDeviceSynchronize());
+  return PU_OK;
+}
+
 int dycore_init_device(pu_ctx *ctx, const pu_coupler_state *cs) {
   const pu::Grid &g = ctx->g;
   const pu_config &c = ctx->cfg;

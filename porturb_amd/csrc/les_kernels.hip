@@ -155,7 +155,7 @@ __device__ __forceinline__ void face_flux(const double *__restrict__ F, const Le
 
 __global__ __launch_bounds__(256) void k_les_update(const double *__restrict__ F, double *uu, double *vv, double *ww,
                                                     double *tt, TracerPtrs trc, LesParams P) {
-  long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long id = (long long)xcd_swizzle(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (id >= P.n3) return;
   int i = (int)(id % P.nx), j = (int)((id / P.nx) % P.ny), k = (int)(id / ((long long)P.nx * P.ny));
   int ih = i + 1, jh = j + 1, kh = k + 1;

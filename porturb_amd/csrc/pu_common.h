@@ -42,6 +42,7 @@ struct pu_ctx {
   int nv = 0;                      // 5 + num_tracers  (conserved variables)
   int nf = 0;                      // nv + 1           (reconstructed fields incl. pressure)
   bool dycore_ready = false;
+  bool les_ready = false;
   // hydrostatic background, FP64 (host + device copies)
   std::vector<double> hy_cells[3];     // dens, theta, pressure: [nz+2hs]
   std::vector<double> hy_edges[3];     // [nz+1]
@@ -80,6 +81,27 @@ int  fail(int code, const std::string &msg);
       return ::pu::fail(PU_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
   } while (0)
 
+// XCD-aware block order.  The dispatcher hands workgroups to the 8 XCDs of an MI355X round-robin
+// by linear workgroup id, and each XCD has its own L2; blocks that share cache lines (x-neighbours)
+// should therefore run on the same XCD.  This maps the hardware id so that consecutive logical
+// ids land on one XCD: hardware id h (XCD h % 8, slot h / 8) -> logical id
+// start(XCD) + slot, a bijection on [0, nblocks).
+__device__ __forceinline__ unsigned xcd_swizzle(unsigned h, unsigned nblocks) {
+  const unsigned nx = 8, q = nblocks / nx, r = nblocks % nx;
+  const unsigned xcd = h % nx, slot = h / nx;
+  return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+// Logical 3-D block index of the current workgroup after the swizzle (x fastest).
+__device__ __forceinline__ void swizzled_block(unsigned &bx, unsigned &by, unsigned &bz) {
+  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned h = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned l = xcd_swizzle(h, nb);
+  bx = l % gridDim.x;
+  by = (l / gridDim.x) % gridDim.y;
+  bz = l / (gridDim.x * gridDim.y);
+}
+
 // Tracer pointer bundles passed by value to kernels.
 struct TracerPtrs { double *p[PU_MAX_TRACERS]; };
 struct TracerPtrsC { const double *p[PU_MAX_TRACERS]; };
@@ -112,6 +134,9 @@ int launch_convert_d2c(pu_ctx *ctx, pu_coupler_state *cs, hipStream_t s);
 int launch_tend_export(pu_ctx *ctx, double *tend, hipStream_t s);
 int dycore_init_device(pu_ctx *ctx, const pu_coupler_state *cs);
 int les_apply_device(pu_ctx *ctx, pu_coupler_state *cs, double dt);
+int les_init_device(pu_ctx *ctx, const pu_coupler_state *cs);
+int fill_device(double *ptr, double value, size_t count);
+int copy_device(void *dst, const void *src, size_t bytes);
 int diagnostics_device(pu_ctx *ctx, const pu_coupler_state *cs, double *mass, double *maxw, double *maxabsw);
 int case_init_device(const pu_config *cfg, int case_id, unsigned long long seed, pu_coupler_state *cs,
                      double *imm, hipStream_t s);
