@@ -162,6 +162,69 @@ int  pu_driver_tracer_name(pu_driver *d, int index, char *buf, int buflen);
 const char *pu_driver_last_error(void);
 int  pu_driver_destroy(pu_driver *d);
 
+/* ------------------------------------------------------------------------------------------
+ * Learned subgrid closure.  No counterpart in the reference (its nearest prior art is the
+ * explicit-vs-closure SGS comparison in build/postproc/compare_les.py:11-60 and the empty
+ * external/ponni inference submodule, model/CMakeLists.txt:22).  A per-cell network
+ * 108 -> 64 -> 64 -> 9 (GELU) maps the differences of (u, v, w, theta) over a cell's 3x3x3
+ * neighbourhood to the subgrid stresses tau_11 tau_12 tau_13 tau_22 tau_23 tau_33 and heat
+ * fluxes q_1 q_2 q_3; attached to a context it replaces LES_Closure's eddy-viscosity momentum and
+ * theta face fluxes (les_closure.h:92-229) on every face whose two cells hold the network's flag,
+ * and keeps them elsewhere (the fallback: wall levels, immersed neighbourhoods, inputs outside the
+ * training range, negative predicted dissipation, effective viscosity above the closure's cap).
+ * Training runs on the device: BF16 matrix-core forward/backward with FP32 accumulation,
+ * FP32 master weights, Adam.
+ * ------------------------------------------------------------------------------------------ */
+#define PU_SGS_NIN 108
+#define PU_SGS_NHIDDEN 64
+#define PU_SGS_NOUT 9
+/* parameter vector: W1[64][108], b1[64], W2[64][64], b2[64], W3[9][64], b3[9] (row = output) */
+#define PU_SGS_NPARAM 11721
+typedef struct pu_mlp pu_mlp;
+
+/* Network with zero parameters and unit normalisation. */
+int  pu_mlp_create(pu_mlp **out);
+int  pu_mlp_destroy(pu_mlp *m);
+/* Host parameter vectors (PU_SGS_NPARAM floats).  set_params also resets the Adam state. */
+int  pu_mlp_set_params(pu_mlp *m, const float *params);
+int  pu_mlp_get_params(const pu_mlp *m, float *params);
+/* Input/output normalisation: input f is (neighbour - centre) * feat_scale[f] / delta;
+ * |input| > xmax marks a cell out of range; tau_o = out_mean[o] + out_scale[o] * y_o. */
+int  pu_mlp_set_normalisation(pu_mlp *m, const float *feat_scale, float xmax, const float *out_mean,
+                              const float *out_scale);
+int  pu_mlp_get_normalisation(const pu_mlp *m, float *feat_scale, float *xmax, float *out_mean,
+                              float *out_scale);
+int  pu_mlp_save(const pu_mlp *m, const char *path);
+int  pu_mlp_load(const char *path, pu_mlp **out);
+/* Device inputs: X [N][112] bfloat16 bit patterns (inputs 108..111 zero), T [N][9] float
+ * (normalised targets), idx [n] int sample rows of this batch (NULL: rows 0..n-1). */
+int  pu_mlp_pack_inputs(const float *X, long long n, unsigned short *Xbf);   /* [n][108] f32 -> bf16 */
+int  pu_mlp_forward(pu_mlp *m, const unsigned short *Xbf, const int *idx, int n, float *Y);   /* Y [n][9] */
+/* Mean-squared loss over the batch and its gradient (device out, PU_SGS_NPARAM floats). */
+int  pu_mlp_loss_grad(pu_mlp *m, const unsigned short *Xbf, const float *T, const int *idx, int n,
+                      float *grad, float *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the batch gradient; loss optional. */
+int  pu_mlp_train_step(pu_mlp *m, const unsigned short *Xbf, const float *T, const int *idx, int n, float lr,
+                       float *loss);
+/* One Adam step with a given device gradient (PU_SGS_NPARAM floats). */
+int  pu_mlp_adam_apply(pu_mlp *m, const float *grad, float lr);
+
+/* Box filter of a fine run held by `fine` (its config and immersed_proportion) by r in each
+ * direction: les [6][nz/r+2][ny/r+2][nx/r+2] double (rho u v w theta k_sgs, halo 1: x/y periodic,
+ * z copies with w = 0), imm [nz/r+2][ny/r+2][nx/r+2] double (block max, z halos 1),
+ * targets [9][nz/r][ny/r][nx/r] float (tau_ij, q_j).  Device pointers. */
+int  pu_sgs_filter(pu_ctx *fine, const pu_coupler_state *cs, int r, double *les, double *imm, float *targets);
+/* Inputs of the listed cells (interior ids of a halo-1 box les [>=5][nz+2][ny+2][nx+2]):
+ * Xbf [n][112] bfloat16; fmul [108] host multipliers (feat_scale / delta). */
+int  pu_sgs_features(const double *les, int nx, int ny, int nz, const float *fmul, const long long *cells,
+                     long long n, unsigned short *Xbf);
+/* Attach (m != NULL) or detach the network to a context's LES closure.  The context reads the
+ * network's current parameters at every apply. */
+int  pu_les_closure_set_model(pu_ctx *ctx, const pu_mlp *m);
+/* The network's prediction for the coupler state: tau [9][nz][ny][nx] float and flags
+ * [nz][ny][nx] (bit 0 momentum, bit 1 heat from the network), device outputs. */
+int  pu_les_closure_sgs(pu_ctx *ctx, const pu_coupler_state *cs, double dt_phys, float *tau, unsigned char *flags);
+
 /* Library build information (gfx target, precision variants compiled). */
 const char *pu_build_info(void);
 

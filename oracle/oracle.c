@@ -833,7 +833,25 @@ double oc_total_mass(const oc_config *c, const double *cp) {
 /* LES closure (les_closure.h:30-536)                                                        */
 /* ----------------------------------------------------------------------------------------- */
 
+static int les_apply_impl(const oc_config *c, double *cp, const double *immersed_prop, double dtphys,
+                          const oc_sgs_model *M, int bf16, float *tau_out, unsigned char *flags_out);
+
 int oc_les_apply(const oc_config *c, double *cp, const double *immersed_prop, double dtphys) {
+  return les_apply_impl(c, cp, immersed_prop, dtphys, NULL, 0, NULL, NULL);
+}
+
+int oc_les_apply_learned(const oc_config *c, double *cp, const double *immersed_prop, double dtphys,
+                         const oc_sgs_model *M, int bf16, float *tau_out, unsigned char *flags_out) {
+  if (!M) return -1;
+  return les_apply_impl(c, cp, immersed_prop, dtphys, M, bf16, tau_out, flags_out);
+}
+
+/* The learned closure (no reference counterpart) replaces, on every face whose two cells both
+ * hold the network's flag, the eddy-viscosity momentum fluxes by rho_face * avg(tau_i,dir) and
+ * the theta flux by rho_face * avg(q_dir); TKE, tracers and all other faces keep
+ * les_closure.h:92-229. */
+static int les_apply_impl(const oc_config *c, double *cp, const double *immersed_prop, double dtphys,
+                          const oc_sgs_model *M, int bf16, float *tau_out, unsigned char *flags_out) {
   set_threads(c);
   const int hs = 1;
   int nx = c->nx, ny = c->ny, nz = c->nz, ntr_all = c->num_tracers;
@@ -1067,6 +1085,49 @@ int oc_les_apply(const oc_config *c, double *cp, const double *immersed_prop, do
       }
     }
   }
+  float *tau = NULL;
+  unsigned char *flg = NULL;
+  if (M) {
+    tau = (float *)malloc(sizeof(float) * 9 * n3);
+    flg = (unsigned char *)malloc(n3);
+    double nu_max = 0.5 * fmin(visc_max_x, fmin(visc_max_y, visc_max_z));
+    oc_sgs_infer(M, state, imm, nx, ny, nz, dx, dy, dz, nu_max, bf16, tau, flg);
+    /* tau component of momentum field f (0 u, 1 v, 2 w) on a face normal to direction d */
+    static const int TIDX[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    int px = c->bc_x == OC_BC_PERIODIC, py = c->bc_y == OC_BC_PERIODIC, pz = c->bc_z == OC_BC_PERIODIC;
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < nz + 1; k++) for (int j = 0; j < ny + 1; j++) for (int i = 0; i < nx + 1; i++) {
+      for (int d = 0; d < 3; d++) {
+        int ok = d == 0 ? (j < ny && k < nz) : d == 1 ? (i < nx && k < nz) : (i < nx && j < ny);
+        if (!ok) continue;
+        int im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2), ic = i, jc = j, kc = k;
+        int n_ = d == 0 ? nx : d == 1 ? ny : nz, per = d == 0 ? px : d == 1 ? py : pz;
+        int lo = d == 0 ? im : d == 1 ? jm : km, hi = d == 0 ? ic : d == 1 ? jc : kc;
+        if (lo < 0 || hi >= n_) {
+          if (!per) continue;
+          if (lo < 0) { if (d == 0) im = nx - 1; else if (d == 1) jm = ny - 1; else km = nz - 1; }
+          if (hi >= n_) { if (d == 0) ic = 0; else if (d == 1) jc = 0; else kc = 0; }
+        }
+        size_t a = ((size_t)km * ny + jm) * nx + im, b = ((size_t)kc * ny + jc) * nx + ic;
+        int both = flg[a] & flg[b];
+        if (!both) continue;
+        /* rho on the face, from the halo box as the eddy-viscosity flux uses it */
+        double rho;
+        if (d == 0) rho = 0.5 * (S_(0, hs + k, hs + j, hs + i - 1) + S_(0, hs + k, hs + j, hs + i));
+        else if (d == 1) rho = 0.5 * (S_(0, hs + k, hs + j - 1, hs + i) + S_(0, hs + k, hs + j, hs + i));
+        else rho = 0.5 * (S_(0, hs + k - 1, hs + j, hs + i) + S_(0, hs + k, hs + j, hs + i));
+        for (int f = 0; f < 4; f++) {
+          if (f < 3 && !(both & 1)) continue;
+          if (f == 3 && !(both & 2)) continue;
+          int t = f < 3 ? TIDX[f][d] : 6 + d;
+          double v = rho * (0.5 * ((double)tau[t * n3 + a] + (double)tau[t * n3 + b]));
+          if (d == 0) FX(f, k, j, i) = v; else if (d == 1) FY(f, k, j, i) = v; else FZ(f, k, j, i) = v;
+        }
+      }
+    }
+    if (tau_out) memcpy(tau_out, tau, sizeof(float) * 9 * n3);
+    if (flags_out) memcpy(flags_out, flg, n3);
+  }
   /* TKE source (:233-278) and forward-Euler update (:280-320), then convert back (:375-414) */
 #pragma omp parallel for collapse(2)
   for (int k = 0; k < nz; k++) for (int j = 0; j < ny; j++) for (int i = 0; i < nx; i++) {
@@ -1132,6 +1193,6 @@ int oc_les_apply(const oc_config *c, double *cp, const double *immersed_prop, do
 #undef FX
 #undef FY
 #undef FZ
-  free(imm); free(state); free(tracers); free(tke); free(fx); free(fy); free(fz);
+  free(imm); free(state); free(tracers); free(tke); free(fx); free(fy); free(fz); free(tau); free(flg);
   return 0;
 }

@@ -89,6 +89,44 @@ int  oc_dycore_immersed_masks(oc_dycore *d, unsigned char *any10, int *tau);
 /* LES_Closure::apply (les_closure.h:30-323), in place on the coupler state. */
 int  oc_les_apply(const oc_config *cfg, double *coupler, const double *immersed_prop, double dt_phys);
 
+/* ----------------------------------------------------------------------------------------- */
+/* Learned subgrid closure (sgs.c; no reference counterpart)                                  */
+/* ----------------------------------------------------------------------------------------- */
+#define OC_MLP_NIN 108
+#define OC_MLP_NH 64
+#define OC_MLP_NOUT 9
+/* parameters: W1[64][108], b1[64], W2[64][64], b2[64], W3[9][64], b3[9] (row = output unit) */
+#define OC_MLP_NPARAM (OC_MLP_NH * OC_MLP_NIN + OC_MLP_NH + OC_MLP_NH * OC_MLP_NH + OC_MLP_NH + OC_MLP_NOUT * OC_MLP_NH + OC_MLP_NOUT)
+
+typedef struct {
+  const float *params;       /* OC_MLP_NPARAM */
+  const float *feat_scale;   /* [108] 1/sigma of each raw input (difference / delta) */
+  float xmax;                /* bound on |normalised input| */
+  const float *out_mean;     /* [9] */
+  const float *out_scale;    /* [9] tau = out_mean + out_scale * y */
+} oc_sgs_model;
+
+float  oc_bf16_round(float x);
+double oc_gelu(double x);
+double oc_gelu_deriv(double x);
+void   oc_mlp_forward(const float *P, const float *X, long n, int bf16, float *Y);
+double oc_mlp_loss_grad(const float *P, const float *X, const float *T, long n, int bf16, double *G);
+void   oc_mlp_adam(float *P, const float *G, float *m, float *v, long np, int t, float lr, float b1, float b2,
+                   float eps);
+/* box filter by r of a fine coupler state: les [6][nz/r+2][ny/r+2][nx/r+2] (rho u v w theta k_sgs,
+ * halo 1), imm_c [nz/r+2][ny/r+2][nx/r+2], targets [9][nz/r][ny/r][nx/r] */
+int    oc_sgs_filter(const oc_config *c, const double *cp, const double *imm_fine, int r, double *les,
+                     double *imm_c, float *targets);
+void   oc_sgs_fmul(const float *feat_scale, double delta, float *fmul);
+int    oc_sgs_features(const double *les, int nx, int ny, int nz, const float *fmul, const long *cells, long n,
+                       float *X);
+void   oc_sgs_infer(const oc_sgs_model *M, const double *les, const double *imm, int nx, int ny, int nz, double dx,
+                    double dy, double dz, double nu_max, int bf16, float *tau, unsigned char *flags);
+/* LES_Closure::apply with the learned momentum/heat fluxes where both cells of a face hold the
+ * network's flag; tau_out [9][n3] and flags_out [n3] optional. */
+int    oc_les_apply_learned(const oc_config *cfg, double *coupler, const double *immersed_prop, double dt_phys,
+                            const oc_sgs_model *M, int bf16, float *tau_out, unsigned char *flags_out);
+
 /* Sum over cells of the dycore's full density rho (dry + mass-adding tracers), times cell volume. */
 double oc_total_mass(const oc_config *cfg, const double *coupler);
 

@@ -33,6 +33,11 @@ class OcConfig(C.Structure):
     ]
 
 
+class OcSgsModel(C.Structure):
+    _fields_ = [("params", C.c_void_p), ("feat_scale", C.c_void_p), ("xmax", C.c_float),
+                ("out_mean", C.c_void_p), ("out_scale", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -69,6 +74,29 @@ def lib():
         L.oc_les_apply.restype = I
         L.oc_total_mass.argtypes = [cp, P]
         L.oc_total_mass.restype = D
+        F, LG = C.c_float, C.c_long
+        L.oc_bf16_round.argtypes = [F]
+        L.oc_bf16_round.restype = F
+        L.oc_gelu.argtypes = [D]
+        L.oc_gelu.restype = D
+        L.oc_gelu_deriv.argtypes = [D]
+        L.oc_gelu_deriv.restype = D
+        L.oc_mlp_forward.argtypes = [P, P, LG, I, P]
+        L.oc_mlp_forward.restype = None
+        L.oc_mlp_loss_grad.argtypes = [P, P, P, LG, I, P]
+        L.oc_mlp_loss_grad.restype = D
+        L.oc_mlp_adam.argtypes = [P, P, P, P, LG, I, F, F, F, F]
+        L.oc_mlp_adam.restype = None
+        L.oc_sgs_filter.argtypes = [cp, P, P, I, P, P, P]
+        L.oc_sgs_filter.restype = I
+        L.oc_sgs_fmul.argtypes = [P, D, P]
+        L.oc_sgs_fmul.restype = None
+        L.oc_sgs_features.argtypes = [P, I, I, I, P, P, LG, P]
+        L.oc_sgs_features.restype = I
+        L.oc_sgs_infer.argtypes = [C.POINTER(OcSgsModel), P, P, I, I, I, D, D, D, D, I, P, P]
+        L.oc_sgs_infer.restype = None
+        L.oc_les_apply_learned.argtypes = [cp, P, P, D, C.POINTER(OcSgsModel), I, P, P]
+        L.oc_les_apply_learned.restype = I
         _lib = L
     return _lib
 
@@ -164,3 +192,108 @@ def total_mass(cfg, coupler):
     oc = to_oc_config(cfg)
     c = np.ascontiguousarray(coupler, dtype=np.float64)
     return lib().oc_total_mass(C.byref(oc), c.ctypes.data)
+
+
+# --------------------------------------------------------------------------------------------
+# learned subgrid closure (sgs.c)
+# --------------------------------------------------------------------------------------------
+NIN, NH, NOUT = 108, 64, 9
+NPARAM = NH * NIN + NH + NH * NH + NH + NOUT * NH + NOUT
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def bf16_round(x):
+    return lib().oc_bf16_round(float(x))
+
+
+def mlp_forward(params, X, bf16=True):
+    P, X = _f32(params), _f32(X)
+    n = X.shape[0]
+    Y = np.zeros((n, NOUT), np.float32)
+    lib().oc_mlp_forward(P.ctypes.data, X.ctypes.data, n, int(bf16), Y.ctypes.data)
+    return Y
+
+
+def mlp_loss_grad(params, X, T, bf16=True):
+    P, X, T = _f32(params), _f32(X), _f32(T)
+    G = np.zeros(NPARAM)
+    loss = lib().oc_mlp_loss_grad(P.ctypes.data, X.ctypes.data, T.ctypes.data, X.shape[0], int(bf16), G.ctypes.data)
+    return loss, G
+
+
+def mlp_adam(params, grad, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """In place on params, m, v (float32 arrays)."""
+    g = _f32(grad)
+    for a in (params, m, v):
+        assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    lib().oc_mlp_adam(params.ctypes.data, g.ctypes.data, m.ctypes.data, v.ctypes.data, params.size, t,
+                      lr, b1, b2, eps)
+
+
+def sgs_filter(cfg, coupler, imm, r):
+    oc = to_oc_config(cfg)
+    nxc, nyc, nzc = cfg["nx"] // r, cfg["ny"] // r, cfg["nz"] // r
+    les = np.zeros((6, nzc + 2, nyc + 2, nxc + 2))
+    immc = np.zeros((nzc + 2, nyc + 2, nxc + 2))
+    tg = np.zeros((9, nzc, nyc, nxc), np.float32)
+    c = np.ascontiguousarray(coupler, dtype=np.float64)
+    im = np.ascontiguousarray(imm, dtype=np.float64)
+    assert lib().oc_sgs_filter(C.byref(oc), c.ctypes.data, im.ctypes.data, r, les.ctypes.data, immc.ctypes.data,
+                               tg.ctypes.data) == 0
+    return les, immc, tg
+
+
+def sgs_fmul(feat_scale, delta):
+    fs = _f32(feat_scale)
+    out = np.zeros(NIN, np.float32)
+    lib().oc_sgs_fmul(fs.ctypes.data, delta, out.ctypes.data)
+    return out
+
+
+def sgs_features(les, fmul, cells):
+    les = np.ascontiguousarray(les, dtype=np.float64)
+    nz, ny, nx = les.shape[1] - 2, les.shape[2] - 2, les.shape[3] - 2
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    fm = _f32(fmul)
+    X = np.zeros((cells.size, NIN), np.float32)
+    assert lib().oc_sgs_features(les.ctypes.data, nx, ny, nz, fm.ctypes.data, cells.ctypes.data, cells.size,
+                                 X.ctypes.data) == 0
+    return X
+
+
+class SgsModel:
+    """Parameters and normalisation of a learned closure, as the oracle reads them."""
+
+    def __init__(self, params, feat_scale, xmax, out_mean, out_scale):
+        self.params, self.feat_scale = _f32(params), _f32(feat_scale)
+        self.out_mean, self.out_scale = _f32(out_mean), _f32(out_scale)
+        self.xmax = float(xmax)
+        self.s = OcSgsModel(self.params.ctypes.data, self.feat_scale.ctypes.data, self.xmax,
+                            self.out_mean.ctypes.data, self.out_scale.ctypes.data)
+
+
+def sgs_infer(model, les, imm, dx, dy, dz, nu_max, bf16=True):
+    les = np.ascontiguousarray(les, dtype=np.float64)
+    imm = np.ascontiguousarray(imm, dtype=np.float64)
+    nz, ny, nx = les.shape[1] - 2, les.shape[2] - 2, les.shape[3] - 2
+    tau = np.zeros((9, nz, ny, nx), np.float32)
+    flags = np.zeros((nz, ny, nx), np.uint8)
+    lib().oc_sgs_infer(C.byref(model.s), les.ctypes.data, imm.ctypes.data, nx, ny, nz, dx, dy, dz, nu_max,
+                       int(bf16), tau.ctypes.data, flags.ctypes.data)
+    return tau, flags
+
+
+def les_apply_learned(cfg, coupler, imm, dt, model, bf16=True, nthreads=0):
+    """LES closure step with the learned fluxes; returns (tau, flags) of the cells."""
+    oc = to_oc_config(cfg, nthreads)
+    imm = np.ascontiguousarray(imm, dtype=np.float64)
+    assert coupler.flags["C_CONTIGUOUS"] and coupler.dtype == np.float64
+    shape = (cfg["nz"], cfg["ny"], cfg["nx"])
+    tau = np.zeros((9,) + shape, np.float32)
+    flags = np.zeros(shape, np.uint8)
+    assert lib().oc_les_apply_learned(C.byref(oc), coupler.ctypes.data, imm.ctypes.data, dt, C.byref(model.s),
+                                      int(bf16), tau.ctypes.data, flags.ctypes.data) == 0
+    return tau, flags

@@ -61,7 +61,8 @@ template <class P> static int alloc(P **p, size_t bytes) {
 
 static void free_ctx(pu_ctx *c) {
   void *ptrs[] = {c->d_hy, c->d_imm_flag, c->d_any10, c->d_tau, c->d_relax, c->d_imm_les, c->d_imm_les_val,
-                  c->d_U, c->d_Us, c->d_Ulo, c->d_Uslo, c->d_tend, c->d_prim, c->d_les, c->d_red};
+                  c->d_U, c->d_Us, c->d_Ulo, c->d_Uslo, c->d_tend, c->d_prim, c->d_les, c->d_red,
+                  c->d_sgs_tau, c->d_sgs_flags, c->d_sgs_valid, c->d_sgs_fmul};
   for (void *p : ptrs) if (p) hipFree(p);
   if (c->t0) hipEventDestroy(c->t0);
   if (c->t1) hipEventDestroy(c->t1);
@@ -150,6 +151,7 @@ int pu_dycore_init(pu_ctx *ctx, const pu_coupler_state *cs) {
   rc = dycore_init_device(ctx, cs);
   if (rc) return rc;
   ctx->dycore_ready = true;
+  ctx->sgs_valid_ready = false;
   return PU_OK;
 }
 
@@ -227,6 +229,7 @@ int pu_les_closure_init(pu_ctx *ctx, const pu_coupler_state *cs) {
   rc = les_init_device(ctx, cs);
   if (rc) return rc;
   ctx->les_ready = true;
+  ctx->sgs_valid_ready = false;
   return PU_OK;
 }
 

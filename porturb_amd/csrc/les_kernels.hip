@@ -21,6 +21,9 @@ struct LesParams {
   double visc_max_x, visc_max_y, visc_max_z;
   double R_d, C0, gamma;
   const double *imm;               // immersed_proportion_halos [nzh][nyh][nxh]
+  // learned closure: per-cell stresses/heat fluxes [9][n3] and flags [n3] (null: eddy viscosity only)
+  const float *tau;
+  const unsigned char *sflags;
 };
 
 __device__ __forceinline__ int wrapl(int i, int n) { i %= n; return i < 0 ? i + n : i; }
@@ -151,6 +154,33 @@ __device__ __forceinline__ void face_flux(const double *__restrict__ F, const Le
   }
   fc.f[3] = -rho * visc_tot_th * (dtn);
   fc.f[4] = -rho * visc_tot * 2 * (dKn);
+  if (P.sflags) {
+    // Learned closure (no reference counterpart): on a face whose two cells both hold the
+    // network's flag, the momentum fluxes become rho * avg(tau_{i,DIR}) (bit 0) and the theta
+    // flux rho * avg(q_DIR) (bit 1).  Only the DIR coordinate of the face's cells can leave the
+    // interior; it wraps for periodic boundaries, other faces keep the eddy-viscosity flux.
+    int ic = ih - 1, jc = jh - 1, kc = kh - 1;
+    int im = ic - (DIR == 0), jm = jc - (DIR == 1), km = kc - (DIR == 2);
+    const int n = DIR == 0 ? P.nx : (DIR == 1 ? P.ny : P.nz);
+    const bool per = (DIR == 0 ? P.bc_x : (DIR == 1 ? P.bc_y : P.bc_z)) == PU_BC_PERIODIC;
+    int &lo = DIR == 0 ? im : (DIR == 1 ? jm : km);
+    int &hi = DIR == 0 ? ic : (DIR == 1 ? jc : kc);
+    bool ok = true;
+    if (lo < 0) { if (per) lo += n; else ok = false; }
+    if (hi >= n) { if (per) hi -= n; else ok = false; }
+    if (ok) {
+      const long long a = ((long long)km * P.ny + jm) * P.nx + im, b = ((long long)kc * P.ny + jc) * P.nx + ic;
+      const int both = P.sflags[a] & P.sflags[b];
+      if (both & 1) {
+        for (int f = 0; f < 3; f++) {
+          const int lo_ = f < DIR ? f : DIR, hi_ = f < DIR ? DIR : f;
+          const int t = lo_ == 0 ? hi_ : (lo_ == 1 ? 2 + hi_ : 5);
+          fc.f[f] = rho * (0.5 * ((double)P.tau[t * P.n3 + a] + (double)P.tau[t * P.n3 + b]));
+        }
+      }
+      if (both & 2) fc.f[3] = rho * (0.5 * ((double)P.tau[(6 + DIR) * P.n3 + a] + (double)P.tau[(6 + DIR) * P.n3 + b]));
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_les_update(const double *__restrict__ F, double *uu, double *vv, double *ww,
@@ -230,10 +260,11 @@ __global__ __launch_bounds__(256) void k_les_update(const double *__restrict__ F
   }
 }
 
-int les_apply_device(pu_ctx *ctx, pu_coupler_state *cs, double dt) {
+static int les_params(pu_ctx *ctx, const pu_coupler_state *cs, double dt, LesParams &P, TracerPtrs &tp,
+                      TracerPtrsC &tpc) {
   const pu_config &c = ctx->cfg;
   if (c.idTKE < 0 || c.idTKE >= c.num_tracers) return fail(PU_ERR_ARG, "LES closure needs a TKE tracer (idTKE)");
-  LesParams P;
+  P = LesParams{};
   P.nx = c.nx; P.ny = c.ny; P.nz = c.nz; P.ntr_all = c.num_tracers; P.idTKE = c.idTKE;
   P.bc_x = c.bc_x; P.bc_y = c.bc_y; P.bc_z = c.bc_z;
   P.enable_gravity = c.enable_gravity; P.dns = c.dns;
@@ -249,17 +280,48 @@ int les_apply_device(pu_ctx *ctx, pu_coupler_state *cs, double dt) {
   P.visc_max_x = 0.25 * P.dx * P.dx / dt; P.visc_max_y = 0.25 * P.dy * P.dy / dt; P.visc_max_z = 0.25 * P.dz * P.dz / dt;
   P.R_d = c.R_d; P.C0 = c.C0; P.gamma = c.gamma_d;
   P.imm = ctx->d_imm_les_val;
-  TracerPtrsC tpc;
-  TracerPtrs tp;
+  P.tau = nullptr;
+  P.sflags = nullptr;
   for (int tr = 0; tr < PU_MAX_TRACERS; tr++) {
     tp.p[tr] = tr < c.num_tracers ? cs->tracers[tr] : nullptr;
     tpc.p[tr] = tp.p[tr];
   }
+  return PU_OK;
+}
+
+int les_prim_device(pu_ctx *ctx, const pu_coupler_state *cs) {
+  LesParams P;
+  TracerPtrs tp;
+  TracerPtrsC tpc;
+  int rc = les_params(ctx, cs, 1., P, tp, tpc);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_les_prim, dim3((unsigned)((P.nh + 255) / 256)), dim3(256), 0, ctx->stream, cs->density_dry,
+                     cs->uvel, cs->vvel, cs->wvel, cs->temp, tpc, ctx->d_les, P);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
+int les_apply_device(pu_ctx *ctx, pu_coupler_state *cs, double dt) {
+  LesParams P;
+  TracerPtrs tp;
+  TracerPtrsC tpc;
+  int rc = les_params(ctx, cs, dt, P, tp, tpc);
+  if (rc) return rc;
   {
     LaunchTimer lt(ctx, "les");
     hipLaunchKernelGGL(k_les_prim, dim3((unsigned)((P.nh + 255) / 256)), dim3(256), 0, ctx->stream, cs->density_dry,
                        cs->uvel, cs->vvel, cs->wvel, cs->temp, tpc, ctx->d_les, P);
     PU_HIP_CHECK(hipGetLastError());
+  }
+  if (ctx->sgs_model) {
+    LaunchTimer lt(ctx, "sgs");
+    rc = sgs_infer_device(ctx, dt);
+    if (rc) return rc;
+    P.tau = ctx->d_sgs_tau;
+    P.sflags = ctx->d_sgs_flags;
+  }
+  {
+    LaunchTimer lt(ctx, "les");
     hipLaunchKernelGGL(k_les_update, dim3((unsigned)((P.n3 + 255) / 256)), dim3(256), 0, ctx->stream, ctx->d_les,
                        cs->uvel, cs->vvel, cs->wvel, cs->temp, tp, P);
     PU_HIP_CHECK(hipGetLastError());

@@ -61,6 +61,14 @@ struct pu_ctx {
   void *d_prim = nullptr;                      // [nf][nzh][nyh][nxh]
   // LES work arrays
   double *d_les = nullptr;                     // primitive closure state, halo 1
+  // learned subgrid closure (attached network, its per-cell outputs and the static mask of cells
+  // whose level and 3x3x3 neighbourhood allow it)
+  const pu_mlp *sgs_model = nullptr;
+  float   *d_sgs_tau = nullptr;                // [9][n3]
+  uint8_t *d_sgs_flags = nullptr;              // [n3]
+  uint8_t *d_sgs_valid = nullptr;              // [n3]
+  float   *d_sgs_fmul = nullptr;               // [108]
+  bool     sgs_valid_ready = false;
   // reductions
   double *d_red = nullptr;                     // scratch for reductions
   // timing
@@ -134,6 +142,9 @@ int launch_convert_d2c(pu_ctx *ctx, pu_coupler_state *cs, hipStream_t s);
 int launch_tend_export(pu_ctx *ctx, double *tend, hipStream_t s);
 int dycore_init_device(pu_ctx *ctx, const pu_coupler_state *cs);
 int les_apply_device(pu_ctx *ctx, pu_coupler_state *cs, double dt);
+int les_prim_device(pu_ctx *ctx, const pu_coupler_state *cs);   // closure box d_les only
+// learned closure: prediction and flags into ctx->d_sgs_tau / d_sgs_flags from ctx->d_les
+int sgs_infer_device(pu_ctx *ctx, double dt);
 int les_init_device(pu_ctx *ctx, const pu_coupler_state *cs);
 int fill_device(double *ptr, double value, size_t count);
 int copy_device(void *dst, const void *src, size_t bytes);
