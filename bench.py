@@ -7,7 +7,12 @@ water vapour + TKE + 2 passive tracers, WENO-9, TKE closure, FP64.  One step = o
 Dynamics_Euler_Stratified_WenoFV::time_step (3 stages) + one LES_Closure::apply, inputs resident in
 HBM.  Timed with HIP events on the library's stream.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--config abl_fp64|abl_fp32|bubble]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+                    [--config abl_fp64|abl_fp32|abl_fp32_learned|bubble]
+
+abl_fp32_learned: the FP32 dycore with the learned subgrid closure (network file
+porturb_amd/data/sgs_abl_urban_dx2.bin, trained by porturb_amd.sgs_training) in place of the
+eddy-viscosity momentum/heat fluxes, the TKE closure as its fallback.
 """
 import argparse
 import json
@@ -19,13 +24,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/sec per SSPRK3 step, 1 MI355X; % HBM roofline; err vs FP64"
+SGS_MODEL = os.path.join(ROOT, "porturb_amd", "data", "sgs_abl_urban_dx2.bin")
 
 
 def make_workload(name):
     from porturb_amd import cases, PU_FP64, PU_FP32
     if name == "abl_fp64":
         return cases.abl_urban(precision=PU_FP64), True
-    if name == "abl_fp32":
+    if name in ("abl_fp32", "abl_fp32_learned"):
         return cases.abl_urban(precision=PU_FP32), True
     if name == "bubble":
         return cases.bubble(n=64), False
@@ -89,8 +95,11 @@ def main():
     print("bench: %s, %d cells, warmup %d, steps %d" % (cfg["name"], n3, args.warmup, args.steps),
           file=sys.stderr, flush=True)
     # the drop-in C++ path: core::Coupler + Dynamics_Euler_Stratified_WenoFV (+ LES_Closure)
+    learned = args.config == "abl_fp32_learned"
+    if learned and not os.path.exists(SGS_MODEL):
+        raise SystemExit("bench: %s missing (train it with python -m porturb_amd.sgs_training)" % SGS_MODEL)
     d = pa.Driver(cfg["case_id"], cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlen"] / cfg["nx"], ord=cfg["ord"],
-                  precision=cfg["precision"], seed=1234, closure=les)
+                  precision=cfg["precision"], seed=1234, closure=les, sgs_model=SGS_MODEL if learned else None)
 
     def step():
         d.step(1)
@@ -112,7 +121,7 @@ def main():
     for _ in range(2):
         step()
     d.synchronize()
-    fams = ["weno_x", "weno_y", "weno_z", "prep", "combine", "les"]
+    fams = ["weno_x", "weno_y", "weno_z", "prep", "les", "sgs"]
     times = {f: d.kernel_time(f) for f in fams}
     d.kernel_timing(False)
     dom = max(["weno_x", "weno_y", "weno_z"], key=lambda f: times[f][0])
@@ -144,8 +153,10 @@ def main():
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64" if fp64 else "f32", "data": "synthetic",
-           "config": {"workload": cfg["name"], "cells": n3, "ord": cfg["ord"], "path": "C++ Coupler + modules",
-                      "closure": "tke" if les else "none", "tracers": cfg["num_tracers"],
+           "config": {"workload": cfg["name"] + ("_learned" if learned else ""), "cells": n3, "ord": cfg["ord"],
+                      "path": "C++ Coupler + modules",
+                      "closure": ("learned (bf16 matrix cores) + tke fallback" if learned else "tke") if les else "none",
+                      "tracers": cfg["num_tracers"],
                       "step": "dycore.time_step + les_closure.apply" if les else "dycore.time_step"},
            "roofline": roof}
     d.destroy()

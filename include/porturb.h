@@ -151,10 +151,11 @@ int  pu_kernel_time(pu_ctx *ctx, const char *name, double *total_ms, long long *
  * experiments/examples/abl_neutral.cpp:65-179 main loop: constants as sc_init.h:31-51, tracers
  * water_vapor (+ TKE from LES_Closure::init when closure != 0, + passive_1/2 for the ABL cases),
  * pu_case_init, dycore init; pu_driver_step runs run_module(dycore.time_step) and
- * run_module(les_closure.apply) with dt = compute_time_step, nsteps times. */
+ * run_module(les_closure.apply) with dt = compute_time_step, nsteps times.  sgs_model (NULL or a
+ * network file, with closure != 0) sets option les_sgs_model: the learned subgrid closure. */
 typedef struct pu_driver pu_driver;
 int  pu_driver_create(int case_id, int nx, int ny, int nz, double dx, int ord, int precision,
-                      unsigned long long seed, int closure, pu_driver **out);
+                      unsigned long long seed, int closure, const char *sgs_model, pu_driver **out);
 int  pu_driver_step(pu_driver *d, int nsteps);
 int  pu_driver_state(pu_driver *d, pu_coupler_state *out, int *num_tracers);
 pu_ctx *pu_driver_context(pu_driver *d);

@@ -1,7 +1,9 @@
 // modules::LES_Closure -- drop-in for model/modules/les_closure.h:9-571.
 //
 // init (:19-26) adds the mass-weighted TKE tracer and sets it to 0.1; apply (:30-323) runs the
-// 1.5-order TKE closure in libporturb's kernels on the coupler state.
+// 1.5-order TKE closure in libporturb's kernels on the coupler state.  Option les_sgs_model (a
+// network file written by pu_mlp_save; no reference counterpart) switches on the learned subgrid
+// closure for momentum and theta, with the eddy-viscosity fluxes as its fallback.
 #pragma once
 
 #include "porturb/core/coupler.h"
@@ -34,6 +36,14 @@ struct LES_Closure {
         core::endrun("ERROR: LES_Closure::apply needs the immersed_proportion entry");
       detail::check(pu_les_closure_init(ctx.ctx, &cs));
       ctx.les_ready = true;
+    }
+    if (!ctx.sgs_checked) {
+      ctx.sgs_checked = true;
+      std::string path = coupler.get_option<std::string>("les_sgs_model", "");
+      if (!path.empty()) {
+        detail::check(pu_mlp_load(path.c_str(), &ctx.sgs));
+        detail::check(pu_les_closure_set_model(ctx.ctx, ctx.sgs));
+      }
     }
     detail::check(pu_les_closure_apply(ctx.ctx, &cs, dtphys));
   }

@@ -18,7 +18,12 @@ struct Context {
   pu_ctx *ctx = nullptr;
   std::string signature;      // tracer list + order + precision the context was built for
   bool dycore_ready = false, les_ready = false;
-  ~Context() { if (ctx) pu_destroy(ctx); }
+  pu_mlp *sgs = nullptr;      // learned subgrid closure (option les_sgs_model), if any
+  bool sgs_checked = false;
+  ~Context() {
+    if (ctx) pu_destroy(ctx);
+    if (sgs) pu_mlp_destroy(sgs);
+  }
 };
 
 inline void check(int rc) {

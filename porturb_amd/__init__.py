@@ -90,13 +90,30 @@ def lib():
         "pu_les_closure_init": (I, [P, csp]),
         "pu_fill_device": (I, [P, D, S]),
         "pu_memcpy_d2d": (I, [P, P, S]),
-        "pu_driver_create": (I, [I, I, I, I, D, I, I, C.c_ulonglong, I, C.POINTER(P)]),
+        "pu_driver_create": (I, [I, I, I, I, D, I, I, C.c_ulonglong, I, C.c_char_p, C.POINTER(P)]),
         "pu_driver_step": (I, [P, I]),
         "pu_driver_state": (I, [P, csp, C.POINTER(I)]),
         "pu_driver_context": (P, [P]),
         "pu_driver_tracer_name": (I, [P, I, C.c_char_p, I]),
         "pu_driver_last_error": (C.c_char_p, []),
         "pu_driver_destroy": (I, [P]),
+        "pu_mlp_create": (I, [C.POINTER(P)]),
+        "pu_mlp_destroy": (I, [P]),
+        "pu_mlp_set_params": (I, [P, P]),
+        "pu_mlp_get_params": (I, [P, P]),
+        "pu_mlp_set_normalisation": (I, [P, P, C.c_float, P, P]),
+        "pu_mlp_get_normalisation": (I, [P, P, P, P, P]),
+        "pu_mlp_save": (I, [P, C.c_char_p]),
+        "pu_mlp_load": (I, [C.c_char_p, C.POINTER(P)]),
+        "pu_mlp_pack_inputs": (I, [P, C.c_longlong, P]),
+        "pu_mlp_forward": (I, [P, P, P, I, P]),
+        "pu_mlp_loss_grad": (I, [P, P, P, P, I, P, C.POINTER(C.c_float)]),
+        "pu_mlp_train_step": (I, [P, P, P, P, I, C.c_float, C.POINTER(C.c_float)]),
+        "pu_mlp_adam_apply": (I, [P, P, C.c_float]),
+        "pu_sgs_filter": (I, [P, csp, I, P, P, P]),
+        "pu_sgs_features": (I, [P, I, I, I, P, P, C.c_longlong, P]),
+        "pu_les_closure_set_model": (I, [P, P]),
+        "pu_les_closure_sgs": (I, [P, csp, D, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -280,6 +297,24 @@ class Dycore:
         check(lib().pu_dycore_immersed_masks(self.ctx, any10.ctypes.data, tau.ctypes.data))
         return any10, tau
 
+    def set_sgs_model(self, mlp):
+        check(lib().pu_les_closure_set_model(self.ctx, mlp.h if mlp is not None else None))
+
+    def les_init(self, cs):
+        check(lib().pu_les_closure_init(self.ctx, C.byref(cs.c)))
+
+    def sgs(self, cs, dt):
+        """(tau [9][nz][ny][nx] float32, flags [nz][ny][nx] uint8) of the attached network."""
+        shape = (self.cfg["nz"], self.cfg["ny"], self.cfg["nx"])
+        n3 = int(np.prod(shape))
+        T, Fl = DeviceBuffer(4 * 9 * n3), DeviceBuffer(n3)
+        check(lib().pu_les_closure_sgs(self.ctx, C.byref(cs.c), dt, T.ptr, Fl.ptr))
+        tau = T.download(np.float32, (9,) + shape)
+        fl = Fl.download(np.uint8, shape)
+        T.free()
+        Fl.free()
+        return tau, fl
+
     def diagnostics(self, cs):
         m, w, a = C.c_double(), C.c_double(), C.c_double()
         check(lib().pu_diagnostics(self.ctx, C.byref(cs.c), C.byref(m), C.byref(w), C.byref(a)))
@@ -318,13 +353,141 @@ def case_init(cfg, case_id, seed, cs):
     check(lib().pu_case_init(C.byref(pc), case_id, seed, C.byref(cs.c), cs.immersed.ptr))
 
 
+# ------------------------------------------------------------------------------------------
+# learned subgrid closure
+# ------------------------------------------------------------------------------------------
+SGS_NIN, SGS_NH, SGS_NOUT, SGS_NINP = 108, 64, 9, 112
+SGS_NPARAM = SGS_NH * SGS_NIN + SGS_NH + SGS_NH * SGS_NH + SGS_NH + SGS_NOUT * SGS_NH + SGS_NOUT
+
+
+class Mlp:
+    """A pu_mlp: the 108-64-64-9 network, its normalisation and Adam state, on the device."""
+
+    def __init__(self, params=None, handle=None):
+        if handle is None:
+            p = C.c_void_p()
+            check(lib().pu_mlp_create(C.byref(p)))
+            handle = p.value
+        self.h = handle
+        if params is not None:
+            self.set_params(params)
+
+    @classmethod
+    def load(cls, path):
+        p = C.c_void_p()
+        check(lib().pu_mlp_load(path.encode(), C.byref(p)))
+        return cls(handle=p.value)
+
+    def save(self, path):
+        check(lib().pu_mlp_save(self.h, path.encode()))
+
+    def set_params(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        assert a.size == SGS_NPARAM
+        check(lib().pu_mlp_set_params(self.h, a.ctypes.data))
+
+    def get_params(self):
+        a = np.zeros(SGS_NPARAM, np.float32)
+        check(lib().pu_mlp_get_params(self.h, a.ctypes.data))
+        return a
+
+    def set_normalisation(self, feat_scale, xmax, out_mean, out_scale):
+        fs, om, os_ = (np.ascontiguousarray(x, dtype=np.float32) for x in (feat_scale, out_mean, out_scale))
+        check(lib().pu_mlp_set_normalisation(self.h, fs.ctypes.data, float(xmax), om.ctypes.data, os_.ctypes.data))
+
+    def get_normalisation(self):
+        fs, om, os_ = np.zeros(SGS_NIN, np.float32), np.zeros(SGS_NOUT, np.float32), np.zeros(SGS_NOUT, np.float32)
+        xm = C.c_float()
+        check(lib().pu_mlp_get_normalisation(self.h, fs.ctypes.data, C.byref(xm), om.ctypes.data, os_.ctypes.data))
+        return fs, xm.value, om, os_
+
+    def forward(self, Xbf, n, idx=None):
+        """Xbf: DeviceBuffer [N][112] bf16; returns host [n][9]."""
+        Y = DeviceBuffer(4 * SGS_NOUT * n)
+        check(lib().pu_mlp_forward(self.h, Xbf.ptr, idx.ptr if idx is not None else None, n, Y.ptr))
+        out = Y.download(np.float32, (n, SGS_NOUT))
+        Y.free()
+        return out
+
+    def loss_grad(self, Xbf, T, n, idx=None):
+        G = DeviceBuffer(4 * SGS_NPARAM)
+        loss = C.c_float()
+        check(lib().pu_mlp_loss_grad(self.h, Xbf.ptr, T.ptr, idx.ptr if idx is not None else None, n, G.ptr,
+                                     C.byref(loss)))
+        g = G.download(np.float32, (SGS_NPARAM,))
+        G.free()
+        return loss.value, g
+
+    def train_step(self, Xbf, T, idx, n, lr, want_loss=False):
+        loss = C.c_float()
+        check(lib().pu_mlp_train_step(self.h, Xbf.ptr, T.ptr, idx.ptr if idx is not None else None, n, lr,
+                                      C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def adam_apply(self, grad, lr):
+        G = DeviceBuffer(4 * SGS_NPARAM)
+        G.upload(np.ascontiguousarray(grad, dtype=np.float32))
+        check(lib().pu_mlp_adam_apply(self.h, G.ptr, lr))
+        G.free()
+
+    def destroy(self):
+        if self.h:
+            lib().pu_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def pack_inputs(X):
+    """Host float [n][108] -> device bf16 [n][112] (DeviceBuffer)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n = X.shape[0]
+    dX = DeviceBuffer(X.nbytes)
+    dX.upload(X)
+    out = DeviceBuffer(2 * SGS_NINP * n)
+    check(lib().pu_mlp_pack_inputs(dX.ptr, n, out.ptr))
+    dX.free()
+    return out
+
+
+def sgs_filter(dycore_ctx, cs_struct, cfg, r):
+    """Box filter of a fine state: device buffers (les, imm, targets) of the coarse grid."""
+    nxc, nyc, nzc = cfg["nx"] // r, cfg["ny"] // r, cfg["nz"] // r
+    nh = (nxc + 2) * (nyc + 2) * (nzc + 2)
+    les, imm, tg = DeviceBuffer(8 * 6 * nh), DeviceBuffer(8 * nh), DeviceBuffer(4 * 9 * nxc * nyc * nzc)
+    check(lib().pu_sgs_filter(dycore_ctx, C.byref(cs_struct), r, les.ptr, imm.ptr, tg.ptr))
+    return les, imm, tg
+
+
+def sgs_features(les, nx, ny, nz, fmul, cells):
+    """Inputs of the listed cells of a halo-1 box (DeviceBuffer) -> DeviceBuffer bf16 [n][112]."""
+    cells = np.ascontiguousarray(cells, dtype=np.int64)
+    fm = np.ascontiguousarray(fmul, dtype=np.float32)
+    dc = DeviceBuffer(cells.nbytes)
+    dc.upload(cells)
+    out = DeviceBuffer(2 * SGS_NINP * cells.size)
+    check(lib().pu_sgs_features(les.ptr, nx, ny, nz, fm.ctypes.data, dc.ptr, cells.size, out.ptr))
+    dc.free()
+    return out
+
+
+def bf16_to_float(a):
+    """uint16/int16 bfloat16 bit patterns -> float32."""
+    return (np.asarray(a).astype(np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
 class Driver:
     """The C++ benchmark driver (core::Coupler + Dynamics_Euler_Stratified_WenoFV + LES_Closure)
     for one case; the coupler's DataManager owns the state."""
 
-    def __init__(self, case_id, nx, ny, nz, dx, ord=9, precision=PU_FP64, seed=1234, closure=True):
+    def __init__(self, case_id, nx, ny, nz, dx, ord=9, precision=PU_FP64, seed=1234, closure=True, sgs_model=None):
         p = C.c_void_p()
-        rc = lib().pu_driver_create(case_id, nx, ny, nz, dx, ord, precision, seed, int(closure), C.byref(p))
+        rc = lib().pu_driver_create(case_id, nx, ny, nz, dx, ord, precision, seed, int(closure),
+                                    sgs_model.encode() if sgs_model else None, C.byref(p))
         if rc != PU_OK:
             raise RuntimeError("pu_driver_create: %s %s" % (lib().pu_driver_last_error().decode(),
                                                           lib().pu_last_error().decode()))

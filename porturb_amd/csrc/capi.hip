@@ -179,10 +179,10 @@ int pu_dycore_time_step(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys) {
       const void *U = stage == 0 ? ctx->d_U : ctx->d_Us;
       const void *Ulo = stage == 0 ? ctx->d_Ulo : ctx->d_Uslo;
       if ((rc = launch_prep(ctx, U, Ulo, s))) return rc;
-      if ((rc = launch_sweeps(ctx, s))) return rc;
       // the final relaxation of a step is followed by the opening relaxation of the next one
       int nrelax = (stage == 2 && cyc + 1 < ncycles) ? 2 : 1;
-      if ((rc = launch_combine(ctx, stage, dt_dyn, nrelax, s))) return rc;
+      // x, y sweeps accumulate tendencies; the z sweep finishes them and combines the stage
+      if ((rc = launch_sweeps(ctx, s, stage, dt_dyn, nrelax))) return rc;
     }
   }
   if ((rc = launch_convert_d2c(ctx, cs, s))) return rc;

@@ -27,7 +27,7 @@ static thread_local std::string g_driver_error;
 extern "C" {
 
 int pu_driver_create(int case_id, int nx, int ny, int nz, double dx, int ord, int precision,
-                     unsigned long long seed, int closure, pu_driver **out) {
+                     unsigned long long seed, int closure, const char *sgs_model, pu_driver **out) {
   if (!out) return PU_ERR_ARG;
   *out = nullptr;
   try {
@@ -47,6 +47,7 @@ int pu_driver_create(int case_id, int nx, int ny, int nz, double dx, int ord, in
     c.set_option<int>("porturb_ord", ord);
     c.set_option<std::string>("dycore_precision", precision == PU_FP32 ? "fp32" : "fp64");
     c.set_option<bool>("function_timers", false);
+    if (sgs_model && sgs_model[0] && closure) c.set_option<std::string>("les_sgs_model", sgs_model);
     c.distribute_mpi_and_allocate_coupled_state(nz, (size_t)ny, (size_t)nx);
     c.set_grid(nx * dx, ny * dx, nz * dx);
     auto &dm = c.get_data_manager_readwrite();

@@ -7,8 +7,9 @@
 //   sweep x/y/z: WENO reconstruction of every field for every cell of a line segment, the
 //               acoustic upwind Riemann flux at each face and the flux divergence, with the face
 //               states exchanged through LDS -- no edge arrays in HBM (:572-728, :808-912);
-//   combine   : gravity/Coriolis sources, the SSPRK stage combination, positive-tracer clipping and
-//               the immersed-boundary relaxation (:308-362, :370-473).
+//   the z sweep, as it finishes each cell's tendencies, also applies the gravity/Coriolis sources,
+//   the SSPRK stage combination, positive-tracer clipping and the immersed-boundary relaxation
+//   (:308-362, :370-473), so the finished tendencies never return to HBM.
 // Layout: every field SoA [var][z][y][x], x fastest.  The state is double (PU_FP64) or float with a
 // float compensation term for rho and rho*theta (PU_FP32).
 #include "pu_common.h"
@@ -279,16 +280,85 @@ __device__ __forceinline__ void weno_edges(const T *s, bool do_map, bool imm_L, 
 }
 
 // ------------------------------------------------------------------------------------------------
+// SSPRK3 stage combination of one conserved field of one cell (time_step_rk_3_3,
+// dynamics_rk.h:279-367): gravity and Coriolis sources, the stage's convex combination, the
+// positivity clip of the last stage and nrelax immersed relaxations (enforce_immersed_boundaries,
+// :370-473).  rho_in, u_in, v_in are the stage-input state of the cell, read before any field of the
+// cell is overwritten (stage 1 writes Us in place).
+// ------------------------------------------------------------------------------------------------
+template <class ST> struct FuseArgs {
+  ST *U, *Ulo, *Us, *Uslo;
+  CombineParams C;
+};
+
+template <class ST>
+__device__ __forceinline__ void combine_field(const FuseArgs<ST> &F, int l, long long id, double tn, double rho_in,
+                                              double u_in, double v_in, double mult, double tr_r, double tr_rt) {
+  const CombineParams &P = F.C;
+  if (l == idW && P.enable_gravity) tn += -P.grav * (rho_in - tr_r);
+  if (P.coriolis && l == idU) tn += P.fcor * v_in;
+  if (P.coriolis && l == idV) tn -= P.fcor * u_in;
+  double un = load_cons(F.U, F.Ulo, l, id, P.n3);
+  double v;
+  if (P.stage == 0) {
+    v = un + P.dt * tn;
+  } else if (P.stage == 1) {
+    double us = load_cons(F.Us, F.Uslo, l, id, P.n3);
+    v = (3. / 4.) * un + (1. / 4.) * us + (1. / 4.) * P.dt * tn;
+  } else {
+    double us = load_cons(F.Us, F.Uslo, l, id, P.n3);
+    v = (1. / 3.) * un + (2. / 3.) * us + (2. / 3.) * P.dt * tn;
+  }
+  bool positive = l >= NUM_STATE && ((P.positive_mask >> (l - NUM_STATE)) & 1u);
+  if (P.stage == 2 && positive) v = fmax(0., v);
+  double target = l == idR ? tr_r : (l == idT ? tr_rt : 0.);
+  for (int r = 0; r < P.nrelax; r++) {
+    v = v + (target - v) * mult;
+    if (positive) v = fmax(0., v);
+  }
+  if (P.stage == 2) store_cons(F.U, F.Ulo, l, id, P.n3, v);
+  else store_cons(F.Us, F.Uslo, l, id, P.n3, v);
+}
+
+template <class ST>
+__device__ __forceinline__ void stage_inputs(const FuseArgs<ST> &F, long long id, int k, double &rho_in, double &u_in,
+                                             double &v_in, double &mult, double &tr_r, double &tr_rt) {
+  const CombineParams &P = F.C;
+  const ST *Sin = P.stage == 0 ? F.U : F.Us;
+  const ST *Sinlo = P.stage == 0 ? F.Ulo : F.Uslo;
+  rho_in = load_cons(Sin, Sinlo, idR, id, P.n3);
+  u_in = P.coriolis ? load_cons(Sin, Sinlo, idU, id, P.n3) : 0.;
+  v_in = P.coriolis ? load_cons(Sin, Sinlo, idV, id, P.n3) : 0.;
+  mult = P.relax[id];
+  tr_r = P.hy_r[P.hs + k];
+  tr_rt = P.hy_r[P.hs + k] * P.hy_t[P.hs + k];
+}
+
+// ------------------------------------------------------------------------------------------------
 // sweep along one direction: reconstruction + Riemann + divergence, face states through LDS.
 // A 256-thread block covers line segments of BS-2 cells along the sweep plus one neighbour cell at
 // each end (the cell whose reconstruction the segment's end faces need); x-contiguous threads keep
 // the loads of y and z sweeps coalesced.  Whole-line blocks (no neighbour cells) were measured
 // slower: 1024-thread blocks leave one block per CU, idle at every barrier.
 // ------------------------------------------------------------------------------------------------
+// Block shapes (BX across, BS along the sweep; BY = BS for y/z).  PU_SEG_{Y,Z}_{BX,BS} override
+// them at compile time for tuning builds.
+#ifndef PU_SEG_Y_BX
+#define PU_SEG_Y_BX 8
+#endif
+#ifndef PU_SEG_Y_BS
+#define PU_SEG_Y_BS 32
+#endif
+#ifndef PU_SEG_Z_BX
+#define PU_SEG_Z_BX 8
+#endif
+#ifndef PU_SEG_Z_BS
+#define PU_SEG_Z_BS 32
+#endif
 template <int DIR> struct SegShape;
 template <> struct SegShape<0> { static constexpr int BX = 64, BY = 4, BS = 64; };
-template <> struct SegShape<1> { static constexpr int BX = 8, BY = 32, BS = 32; };
-template <> struct SegShape<2> { static constexpr int BX = 8, BY = 32, BS = 32; };
+template <> struct SegShape<1> { static constexpr int BX = PU_SEG_Y_BX, BY = PU_SEG_Y_BS, BS = PU_SEG_Y_BS; };
+template <> struct SegShape<2> { static constexpr int BX = PU_SEG_Z_BX, BY = PU_SEG_Z_BS, BS = PU_SEG_Z_BS; };
 
 // acoustic upwind state of one face (dynamics_rk.h:640-651): mass flux, pressure, upwind side
 template <class T> struct Upw { T ru, p; bool left; };
@@ -305,10 +375,12 @@ __device__ __forceinline__ Upw<T> riemann(T rL, T uL, T pL, T rR, T uR, T pR, T 
   return u;
 }
 
-template <int ORD, class T, int DIR>
-__global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ prim, const uint8_t *__restrict__ immf,
+// FUSE (z sweep only): each cell's final tendency goes straight into the stage combination
+// (combine_field) instead of back to `tend`.
+template <int ORD, class T, int DIR, bool FUSE>
+__global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep(const T *__restrict__ prim, const uint8_t *__restrict__ immf,
                                                const uint8_t *__restrict__ any10, T *__restrict__ tend,
-                                               SweepParams P) {
+                                               SweepParams P, FuseArgs<T> FA) {
   constexpr int hs = (ORD - 1) / 2;
   constexpr int BX = SegShape<DIR>::BX, BY = SegShape<DIR>::BY, BS = SegShape<DIR>::BS;
   constexpr int NC = BS - 2;
@@ -386,10 +458,19 @@ __global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ prim, const
 
   const long long cid = ((long long)k * P.ny + j) * P.nx + i;
   const T rd = (T)P.rd;
+  double f_rho = 0, f_u = 0, f_v = 0, f_mult = 0, f_trr = 0, f_trt = 0;
+  if constexpr (FUSE) {
+    if (has_tend) stage_inputs(FA, cid, k, f_rho, f_u, f_v, f_mult, f_trr, f_trt);
+  }
   auto accumulate = [&](int fidx, T v) {
     T *tp = tend + (long long)fidx * P.n3 + cid;
-    if (DIR == 0) *tp = v;
-    else *tp = *tp + v;
+    if constexpr (FUSE) {
+      const T tsum = *tp + v;
+      combine_field(FA, fidx, cid, (double)tsum, f_rho, f_u, f_v, f_mult, f_trr, f_trt);
+    } else {
+      if (DIR == 0) *tp = v;
+      else *tp = *tp + v;
+    }
   };
 
   // hydrostatic add-back: cell values for x/y faces, edge values for z faces (:642-645, :686-689)
@@ -455,52 +536,6 @@ __global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ prim, const
     bufB0[lin] = F;
     __syncthreads();
     if (has_tend) accumulate(fidx, -(bufB0[lin + nbs] - bufB0[lin]) * rd);
-  }
-}
-
-// ------------------------------------------------------------------------------------------------
-// combine: sources + SSPRK3 stage + clip + immersed relaxation
-// ------------------------------------------------------------------------------------------------
-template <class ST>
-__global__ __launch_bounds__(256) void k_combine(ST *__restrict__ U, ST *__restrict__ Ulo, ST *__restrict__ Us,
-                                                 ST *__restrict__ Uslo, const ST *__restrict__ tend,
-                                                 CombineParams P) {
-  long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= P.n3) return;
-  int k = (int)(id / ((long long)P.nx * P.ny));
-  const int nv = NUM_STATE + P.ntr;
-  // stage input: U for stage 0, Us otherwise
-  const ST *Sin = P.stage == 0 ? U : Us;
-  const ST *Sinlo = P.stage == 0 ? Ulo : Uslo;
-  double rho_in = load_cons(Sin, Sinlo, idR, id, P.n3);
-  double mult = P.relax[id];
-  double tr_r = P.hy_r[P.hs + k];
-  double tr_rt = P.hy_r[P.hs + k] * P.hy_t[P.hs + k];
-  for (int l = 0; l < nv; l++) {
-    double tn = (double)tend[(long long)l * P.n3 + id];
-    if (l == idW && P.enable_gravity) tn += -P.grav * (rho_in - P.hy_r[P.hs + k]);
-    if (P.coriolis && l == idU) tn += P.fcor * load_cons(Sin, Sinlo, idV, id, P.n3);
-    if (P.coriolis && l == idV) tn -= P.fcor * load_cons(Sin, Sinlo, idU, id, P.n3);
-    double un = load_cons(U, Ulo, l, id, P.n3);
-    double v;
-    if (P.stage == 0) {
-      v = un + P.dt * tn;
-    } else if (P.stage == 1) {
-      double us = load_cons(Us, Uslo, l, id, P.n3);
-      v = (3. / 4.) * un + (1. / 4.) * us + (1. / 4.) * P.dt * tn;
-    } else {
-      double us = load_cons(Us, Uslo, l, id, P.n3);
-      v = (1. / 3.) * un + (2. / 3.) * us + (2. / 3.) * P.dt * tn;
-    }
-    bool positive = l >= NUM_STATE && ((P.positive_mask >> (l - NUM_STATE)) & 1u);
-    if (P.stage == 2 && positive) v = fmax(0., v);
-    double target = l == idR ? tr_r : (l == idT ? tr_rt : 0.);
-    for (int r = 0; r < P.nrelax; r++) {
-      v = v + (target - v) * mult;
-      if (positive) v = fmax(0., v);
-    }
-    if (P.stage == 2) store_cons(U, Ulo, l, id, P.n3, v);
-    else store_cons(Us, Uslo, l, id, P.n3, v);
   }
 }
 
@@ -616,8 +651,11 @@ int launch_prep(pu_ctx *ctx, const void *U, const void *Ulo, hipStream_t s) {
   return ctx->cfg.precision == PU_FP32 ? prep_t<float>(ctx, U, Ulo, s) : prep_t<double>(ctx, U, Ulo, s);
 }
 
+static CombineParams combine_params(pu_ctx *ctx);
+
+// stage < 0: tendencies only (into ctx->d_tend); otherwise the z sweep also combines stage `stage`
 template <int ORD, class T, int DIR>
-static int sweep_one(pu_ctx *ctx, hipStream_t s) {
+static int sweep_one(pu_ctx *ctx, hipStream_t s, int stage, double dt, int nrelax) {
   const pu::Grid &g = ctx->g;
   SweepParams P;
   P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ntr = ctx->cfg.num_tracers;
@@ -637,35 +675,44 @@ static int sweep_one(pu_ctx *ctx, hipStream_t s) {
   if (DIR == 0) grd = dim3((unsigned)((g.nx + NC - 1) / NC), (unsigned)((g.ny + BY - 1) / BY), (unsigned)g.nz);
   else if (DIR == 1) grd = dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)((g.ny + NC - 1) / NC), (unsigned)g.nz);
   else grd = dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)g.ny, (unsigned)((g.nz + NC - 1) / NC));
-  hipLaunchKernelGGL((k_sweep<ORD, T, DIR>), grd, blk, 0, s, (const T *)ctx->d_prim, ctx->d_imm_flag,
-                     ctx->d_any10, (T *)ctx->d_tend, P);
+  FuseArgs<T> FA{};
+  if (DIR == 2 && stage >= 0) {
+    FA.U = (T *)ctx->d_U; FA.Ulo = (T *)ctx->d_Ulo; FA.Us = (T *)ctx->d_Us; FA.Uslo = (T *)ctx->d_Uslo;
+    FA.C = combine_params(ctx);
+    FA.C.stage = stage; FA.C.dt = dt; FA.C.nrelax = nrelax;
+    hipLaunchKernelGGL((k_sweep<ORD, T, DIR, true>), grd, blk, 0, s, (const T *)ctx->d_prim, ctx->d_imm_flag,
+                       ctx->d_any10, (T *)ctx->d_tend, P, FA);
+  } else {
+    hipLaunchKernelGGL((k_sweep<ORD, T, DIR, false>), grd, blk, 0, s, (const T *)ctx->d_prim, ctx->d_imm_flag,
+                       ctx->d_any10, (T *)ctx->d_tend, P, FA);
+  }
   PU_HIP_CHECK(hipGetLastError());
   return PU_OK;
 }
 
 template <int ORD, class T>
-static int sweeps_t(pu_ctx *ctx, hipStream_t s) {
+static int sweeps_t(pu_ctx *ctx, hipStream_t s, int stage, double dt, int nrelax) {
   int rc;
-  if ((rc = sweep_one<ORD, T, 0>(ctx, s))) return rc;
-  if (ctx->g.ny > 1 && (rc = sweep_one<ORD, T, 1>(ctx, s))) return rc;
-  if ((rc = sweep_one<ORD, T, 2>(ctx, s))) return rc;
+  if ((rc = sweep_one<ORD, T, 0>(ctx, s, stage, dt, nrelax))) return rc;
+  if (ctx->g.ny > 1 && (rc = sweep_one<ORD, T, 1>(ctx, s, stage, dt, nrelax))) return rc;
+  if ((rc = sweep_one<ORD, T, 2>(ctx, s, stage, dt, nrelax))) return rc;
   return PU_OK;
 }
 
-int launch_sweeps(pu_ctx *ctx, hipStream_t s) {
+int launch_sweeps(pu_ctx *ctx, hipStream_t s, int stage, double dt, int nrelax) {
   bool f32 = ctx->cfg.precision == PU_FP32;
   switch (ctx->cfg.ord) {
-    case 3: return f32 ? sweeps_t<3, float>(ctx, s) : sweeps_t<3, double>(ctx, s);
-    case 5: return f32 ? sweeps_t<5, float>(ctx, s) : sweeps_t<5, double>(ctx, s);
-    case 7: return f32 ? sweeps_t<7, float>(ctx, s) : sweeps_t<7, double>(ctx, s);
-    case 9: return f32 ? sweeps_t<9, float>(ctx, s) : sweeps_t<9, double>(ctx, s);
-    case 11: return f32 ? sweeps_t<11, float>(ctx, s) : sweeps_t<11, double>(ctx, s);
+    case 3: return f32 ? sweeps_t<3, float>(ctx, s, stage, dt, nrelax) : sweeps_t<3, double>(ctx, s, stage, dt, nrelax);
+    case 5: return f32 ? sweeps_t<5, float>(ctx, s, stage, dt, nrelax) : sweeps_t<5, double>(ctx, s, stage, dt, nrelax);
+    case 7: return f32 ? sweeps_t<7, float>(ctx, s, stage, dt, nrelax) : sweeps_t<7, double>(ctx, s, stage, dt, nrelax);
+    case 9: return f32 ? sweeps_t<9, float>(ctx, s, stage, dt, nrelax) : sweeps_t<9, double>(ctx, s, stage, dt, nrelax);
+    case 11: return f32 ? sweeps_t<11, float>(ctx, s, stage, dt, nrelax) : sweeps_t<11, double>(ctx, s, stage, dt, nrelax);
   }
   return fail(PU_ERR_UNSUPPORTED, "unsupported ord");
 }
 
 static CombineParams combine_params(pu_ctx *ctx) {
-  CombineParams P;
+  CombineParams P{};
   P.nx = ctx->g.nx; P.ny = ctx->g.ny; P.nz = ctx->g.nz; P.hs = ctx->g.hs; P.ntr = ctx->cfg.num_tracers;
   P.n3 = ctx->g.n3();
   P.enable_gravity = ctx->cfg.enable_gravity;
@@ -678,23 +725,6 @@ static CombineParams combine_params(pu_ctx *ctx) {
   for (int tr = 0; tr < ctx->cfg.num_tracers; tr++) if (ctx->cfg.tracer_positive[tr]) P.positive_mask |= 1u << tr;
   P.stage = 0; P.nrelax = 1; P.dt = 0;
   return P;
-}
-
-template <class ST>
-static int combine_t(pu_ctx *ctx, int stage, double dt, int nrelax, hipStream_t s) {
-  CombineParams P = combine_params(ctx);
-  P.stage = stage; P.nrelax = nrelax; P.dt = dt;
-  long long n = ctx->g.n3();
-  LaunchTimer lt(ctx, "combine");
-  hipLaunchKernelGGL(k_combine<ST>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (ST *)ctx->d_U,
-                     (ST *)ctx->d_Ulo, (ST *)ctx->d_Us, (ST *)ctx->d_Uslo, (const ST *)ctx->d_tend, P);
-  PU_HIP_CHECK(hipGetLastError());
-  return PU_OK;
-}
-
-int launch_combine(pu_ctx *ctx, int stage, double dt, int nrelax, hipStream_t s) {
-  return ctx->cfg.precision == PU_FP32 ? combine_t<float>(ctx, stage, dt, nrelax, s)
-                                       : combine_t<double>(ctx, stage, dt, nrelax, s);
 }
 
 static ConvParams conv_params(pu_ctx *ctx) {

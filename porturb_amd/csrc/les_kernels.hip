@@ -73,16 +73,26 @@ struct FaceCommon {
   double f[5];      // ru rv rw rt tke
 };
 
-// Face flux on the low face of halo'd cell (kh, jh, ih) in direction DIR (les_closure.h:92-229).
-template <int DIR>
-__device__ __forceinline__ void face_flux(const double *__restrict__ F, const LesParams &P, int kh, int jh, int ih,
+// The closure box as the face fluxes read it: field f at box index idx, the immersed proportion,
+// and the box strides.  Tile: the block's cells plus a one-cell halo staged in LDS.
+struct TileBox {
+  const double *sm;
+  int nfl;                                   // fields before the immersed-proportion slot
+  static constexpr int sx = 1, sy = 18, sz = 18 * 6, n = 18 * 6 * 6;
+  __device__ double S(int f, int idx) const { return sm[f * n + idx]; }
+  __device__ double imm(int idx) const { return sm[nfl * n + idx]; }
+};
+
+// Face flux on the low face of box cell c (at halo coordinates kh, jh, ih of the closure box) in
+// direction DIR (les_closure.h:92-229).
+template <int DIR, class Box>
+__device__ __forceinline__ void face_flux(const Box &B, const LesParams &P, int c, int kh, int jh, int ih,
                                           FaceCommon &fc) {
-  const long long sx = 1, sy = P.nxh, sz = P.nxh * P.nyh, nh = P.nh;
-  const long long c = ((long long)kh * P.nyh + jh) * P.nxh + ih;
-  const long long sn = DIR == 0 ? sx : (DIR == 1 ? sy : sz);      // normal stride
-  const long long m = c - sn;                                      // the low neighbour
-  auto S = [&](int f, long long idx) { return F[(long long)f * nh + idx]; };
-  if (P.imm[m] == 1 || P.imm[c] == 1) { fc.zero = true; return; }
+  constexpr int sx = Box::sx, sy = Box::sy, sz = Box::sz;
+  const int sn = DIR == 0 ? sx : (DIR == 1 ? sy : sz);            // normal stride
+  const int m = c - sn;                                            // the low neighbour
+  auto S = [&](int f, int idx) { return B.S(f, idx); };
+  if (B.imm(m) == 1 || B.imm(c) == 1) { fc.zero = true; return; }
   fc.zero = false;
   const double rdx = 1. / P.dx, rdy = 1. / P.dy, rdz = 1. / P.dz;
   const double r2dx = 0.5 * rdx, r2dy = 0.5 * rdy, r2dz = 0.5 * rdz;
@@ -183,22 +193,46 @@ __device__ __forceinline__ void face_flux(const double *__restrict__ F, const Le
   }
 }
 
+constexpr int LTX = 16, LTY = 4, LTZ = 4;       // cells of one block tile (x, y, z)
+
+// One block: a 16x4x4 tile of cells.  The closure box values of the tile plus a one-cell halo
+// (fields rho_d, u, v, w, theta, K, the diffused tracers and the immersed proportion) are staged in
+// LDS once; each thread then evaluates its cell's six face fluxes, the TKE source and the update.
 __global__ __launch_bounds__(256) void k_les_update(const double *__restrict__ F, double *uu, double *vv, double *ww,
                                                     double *tt, TracerPtrs trc, LesParams P) {
-  const long long id = (long long)xcd_swizzle(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (id >= P.n3) return;
-  int i = (int)(id % P.nx), j = (int)((id / P.nx) % P.ny), k = (int)(id / ((long long)P.nx * P.ny));
-  int ih = i + 1, jh = j + 1, kh = k + 1;
-  const long long sx = 1, sy = P.nxh, sz = P.nxh * P.nyh, nh = P.nh;
-  const long long c = ((long long)kh * P.nyh + jh) * P.nxh + ih;
-  auto S = [&](int f, long long idx) { return F[(long long)f * nh + idx]; };
+  extern __shared__ double sm[];
+  const int nfl = 6 + P.nq;
+  unsigned bx, by, bz;
+  swizzled_block(bx, by, bz);
+  const int i0 = bx * LTX, j0 = by * LTY, k0 = bz * LTZ;
+  for (int p = threadIdx.x; p < TileBox::n * (nfl + 1); p += 256) {
+    const int f = p / TileBox::n, q = p % TileBox::n;
+    const int li = q % 18, lj = (q / 18) % 6, lk = q / (18 * 6);
+    const int ih = i0 + li, jh = j0 + lj, kh = k0 + lk;       // closure-box (halo) coordinates
+    double v = 0.;
+    if (ih < P.nxh && jh < P.nyh && kh < P.nzh) {
+      const long long g = ((long long)kh * P.nyh + jh) * P.nxh + ih;
+      v = f < nfl ? F[(long long)f * P.nh + g] : P.imm[g];
+    }
+    sm[p] = v;
+  }
+  __syncthreads();
+  const int li = threadIdx.x % LTX, lj = (threadIdx.x / LTX) % LTY, lk = threadIdx.x / (LTX * LTY);
+  const int i = i0 + li, j = j0 + lj, k = k0 + lk;
+  if (i >= P.nx || j >= P.ny || k >= P.nz) return;
+  const long long id = ((long long)k * P.ny + j) * P.nx + i;
+  const int ih = i + 1, jh = j + 1, kh = k + 1;
+  const TileBox B{sm, nfl};
+  constexpr int sx = TileBox::sx, sy = TileBox::sy, sz = TileBox::sz;
+  const int c = ((lk + 1) * 6 + (lj + 1)) * 18 + (li + 1);
+  auto S = [&](int f, int idx) { return B.S(f, idx); };
   FaceCommon fx0, fx1, fy0, fy1, fz0, fz1;
-  face_flux<0>(F, P, kh, jh, ih, fx0);
-  face_flux<0>(F, P, kh, jh, ih + 1, fx1);
-  face_flux<1>(F, P, kh, jh, ih, fy0);
-  face_flux<1>(F, P, kh, jh + 1, ih, fy1);
-  face_flux<2>(F, P, kh, jh, ih, fz0);
-  face_flux<2>(F, P, kh + 1, jh, ih, fz1);
+  face_flux<0>(B, P, c, kh, jh, ih, fx0);
+  face_flux<0>(B, P, c + sx, kh, jh, ih + 1, fx1);
+  face_flux<1>(B, P, c, kh, jh, ih, fy0);
+  face_flux<1>(B, P, c + sy, kh, jh + 1, ih, fy1);
+  face_flux<2>(B, P, c, kh, jh, ih, fz0);
+  face_flux<2>(B, P, c + sz, kh + 1, jh, ih, fz1);
   const double rdx = 1. / P.dx, rdy = 1. / P.dy, rdz = 1. / P.dz;
   const double r2dx = 0.5 * rdx, r2dy = 0.5 * rdy, r2dz = 0.5 * rdz;
   // TKE source (les_closure.h:233-278)
@@ -209,13 +243,13 @@ __global__ __launch_bounds__(256) void k_les_update(const double *__restrict__ F
   double km = 0.1 * ell * sqrt(K);
   double Pr_t = P.delta / (1 + 2 * ell);
   double src = 0;
-  const double *imm = P.imm;
-  if (imm[c] < 1) {
+  auto imm = [&](int idx) { return B.imm(idx); };
+  if (imm(c) < 1) {
     if (P.enable_gravity) src += -(P.grav * rho * km) / (t * Pr_t) * dt_dz;
     src -= P.diss_mult * rho * (0.19 + 0.51 * ell / P.delta) / P.delta * (K * sqrt(K));
-    long long im1 = imm[c - sx] > 0 ? c : c - sx, ip1 = imm[c + sx] > 0 ? c : c + sx;
-    long long jm1 = imm[c - sy] > 0 ? c : c - sy, jp1 = imm[c + sy] > 0 ? c : c + sy;
-    long long km1 = imm[c - sz] > 0 ? c : c - sz, kp1 = imm[c + sz] > 0 ? c : c + sz;
+    int im1 = imm(c - sx) > 0 ? c : c - sx, ip1 = imm(c + sx) > 0 ? c : c + sx;
+    int jm1 = imm(c - sy) > 0 ? c : c - sy, jp1 = imm(c + sy) > 0 ? c : c + sy;
+    int km1 = imm(c - sz) > 0 ? c : c - sz, kp1 = imm(c + sz) > 0 ? c : c + sz;
     double du_dx = (S(1, c + sx) - S(1, c - sx)) * r2dx;
     double dv_dx = (S(2, ip1) - S(2, im1)) * r2dx;
     double dw_dx = (S(3, ip1) - S(3, im1)) * r2dx;
@@ -248,15 +282,15 @@ __global__ __launch_bounds__(256) void k_les_update(const double *__restrict__ F
   trc.p[P.idTKE][id] = rk;
   // diffused tracers: flux -rho*visc*dq/dn on each open face
   for (int q = 0; q < P.nq; q++) {
-    const double *Q = F + (long long)(6 + q) * nh;
-    double qx0 = fx0.zero ? 0. : -fx0.rho * fx0.visc * ((Q[c] - Q[c - sx]) * rdx);
-    double qx1 = fx1.zero ? 0. : -fx1.rho * fx1.visc * ((Q[c + sx] - Q[c]) * rdx);
-    double qy0 = fy0.zero ? 0. : -fy0.rho * fy0.visc * ((Q[c] - Q[c - sy]) * rdy);
-    double qy1 = fy1.zero ? 0. : -fy1.rho * fy1.visc * ((Q[c + sy] - Q[c]) * rdy);
-    double qz0 = fz0.zero ? 0. : -fz0.rho * fz0.visc * ((Q[c] - Q[c - sz]) * rdz);
-    double qz1 = fz1.zero ? 0. : -fz1.rho * fz1.visc * ((Q[c + sz] - Q[c]) * rdz);
+    auto Q = [&](int idx) { return B.S(6 + q, idx); };
+    double qx0 = fx0.zero ? 0. : -fx0.rho * fx0.visc * ((Q(c) - Q(c - sx)) * rdx);
+    double qx1 = fx1.zero ? 0. : -fx1.rho * fx1.visc * ((Q(c + sx) - Q(c)) * rdx);
+    double qy0 = fy0.zero ? 0. : -fy0.rho * fy0.visc * ((Q(c) - Q(c - sy)) * rdy);
+    double qy1 = fy1.zero ? 0. : -fy1.rho * fy1.visc * ((Q(c + sy) - Q(c)) * rdy);
+    double qz0 = fz0.zero ? 0. : -fz0.rho * fz0.visc * ((Q(c) - Q(c - sz)) * rdz);
+    double qz1 = fz1.zero ? 0. : -fz1.rho * fz1.visc * ((Q(c + sz) - Q(c)) * rdz);
     double tq = -(qx1 - qx0) * rdx - (qy1 - qy0) * rdy - (qz1 - qz0) * rdz;
-    trc.p[P.qidx[q]][id] = Q[c] * rho + dt * tq;
+    trc.p[P.qidx[q]][id] = Q(c) * rho + dt * tq;
   }
 }
 
@@ -322,8 +356,12 @@ int les_apply_device(pu_ctx *ctx, pu_coupler_state *cs, double dt) {
   }
   {
     LaunchTimer lt(ctx, "les");
-    hipLaunchKernelGGL(k_les_update, dim3((unsigned)((P.n3 + 255) / 256)), dim3(256), 0, ctx->stream, ctx->d_les,
-                       cs->uvel, cs->vvel, cs->wvel, cs->temp, tp, P);
+    const dim3 grd((unsigned)((P.nx + LTX - 1) / LTX), (unsigned)((P.ny + LTY - 1) / LTY), (unsigned)((P.nz + LTZ - 1) / LTZ));
+    const size_t lds = sizeof(double) * TileBox::n * (7 + P.nq);     // <= 119 KB for 16 tracers
+    if (lds > 65536)
+      PU_HIP_CHECK(hipFuncSetAttribute((const void *)k_les_update, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_les_update, grd, dim3(256), lds, ctx->stream, ctx->d_les, cs->uvel, cs->vvel, cs->wvel,
+                       cs->temp, tp, P);
     PU_HIP_CHECK(hipGetLastError());
   }
   return PU_OK;

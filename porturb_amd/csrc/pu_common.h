@@ -135,8 +135,9 @@ void collect_timers(pu_ctx *ctx);
 
 // Launchers implemented in the kernel translation units.
 int launch_prep(pu_ctx *ctx, const void *U, const void *Ulo, hipStream_t s);
-int launch_sweeps(pu_ctx *ctx, hipStream_t s);
-int launch_combine(pu_ctx *ctx, int stage, double dt, int nrelax, hipStream_t s);
+// stage < 0: tendencies into d_tend; stage 0..2: the z sweep also applies the stage combination
+// (sources, SSPRK3 stage, clip, nrelax immersed relaxations) for every cell it finishes
+int launch_sweeps(pu_ctx *ctx, hipStream_t s, int stage = -1, double dt = 0., int nrelax = 0);
 int launch_convert_c2d(pu_ctx *ctx, const pu_coupler_state *cs, int relax, hipStream_t s);
 int launch_convert_d2c(pu_ctx *ctx, pu_coupler_state *cs, hipStream_t s);
 int launch_tend_export(pu_ctx *ctx, double *tend, hipStream_t s);

@@ -14,6 +14,9 @@ and the horizontal-mean profiles the reference's analysis plots (build/postproc/
 abl_neutral_analysis.py:122-131): <u>(z), <|U|>(z) and the resolved TKE 1/2<u'^2+v'^2+w'^2>(z),
 primes taken against the horizontal mean of each level, all cells included as there.
 
+With --sgs-model the FP32 run also uses the learned subgrid closure (a network file written by
+pu_mlp_save), and the comparison is FP32 + learned closure against the FP64 plain closure.
+
     python -m porturb_amd.precision_study --nx 1024 --ny 1024 --nz 128 --steps 200 --out res.json
 """
 import argparse
@@ -65,6 +68,17 @@ def compare(cs64, cs32, names):
     return res, prof
 
 
+def learned_fraction(d, cs, dt):
+    """Fractions of cells whose momentum / heat fluxes come from the network for the current
+    state (the attached network's flags)."""
+    n3 = int(np.prod(cs.shape))
+    fl = pa.DeviceBuffer(n3)
+    pa.check(pa.lib().pu_les_closure_sgs(d.ctx, pa.C.byref(cs.c), dt, None, fl.ptr))
+    f = fl.download(np.uint8, (n3,))
+    fl.free()
+    return {"momentum": float(((f & 1) != 0).mean()), "heat": float(((f & 2) != 0).mean())}
+
+
 class State:
     def __init__(self, cfg):
         self.cfg = cfg
@@ -84,6 +98,7 @@ def main():
     ap.add_argument("--steps", type=int, nargs="+", default=[200])
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--sgs-model", default=None)
     args = ap.parse_args()
     cfg64 = cases.abl_urban(nx=args.nx, ny=args.ny, nz=args.nz, precision=pa.PU_FP64)
     cfg32 = cases.abl_urban(nx=args.nx, ny=args.ny, nz=args.nz, precision=pa.PU_FP32)
@@ -93,8 +108,13 @@ def main():
     d64, d32 = pa.Dycore(cfg64), pa.Dycore(cfg32)
     d64.init(s64.cs)
     d32.init(s32.cs)
+    mlp = None
+    if args.sgs_model:
+        mlp = pa.Mlp.load(args.sgs_model)
+        d32.set_sgs_model(mlp)
     names = FIELD_NAMES[:5 + cfg64["num_tracers"]]
-    out = {"workload": cfg64["name"].replace("_fp64", ""), "seed": args.seed, "checkpoints": []}
+    out = {"workload": cfg64["name"].replace("_fp64", ""), "seed": args.seed, "checkpoints": [],
+           "fp32_closure": "learned (%s)" % os.path.basename(args.sgs_model) if args.sgs_model else "tke"}
     step, t0 = 0, time.time()
     for target in sorted(args.steps):
         while step < target:
@@ -113,8 +133,10 @@ def main():
         fields, prof = compare(s64, s32, names)
         m64 = d64.diagnostics(s64.cs)[0]
         m32 = d32.diagnostics(s32.cs)[0]
-        out["checkpoints"].append({"steps": step, "fields": fields, "profiles": prof,
-                                   "mass_rel_diff": abs(m32 - m64) / abs(m64)})
+        ck = {"steps": step, "fields": fields, "profiles": prof, "mass_rel_diff": abs(m32 - m64) / abs(m64)}
+        if mlp is not None:
+            ck["learned_fraction"] = learned_fraction(d32, s32.cs, d64.compute_time_step())
+        out["checkpoints"].append(ck)
         print("precision_study: checkpoint %d done" % step, file=sys.stderr, flush=True)
     d64.destroy()
     d32.destroy()
