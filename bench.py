@@ -2,10 +2,11 @@
 """Benchmark: cell-updates/s of the dycore step (SSPRK3 WENO-FV time_step + LES closure apply) on one
 MI355X, with the dominant kernel's roofline and the CPU oracle timed on the host as a baseline.
 
-Workload (default): neutral urban ABL, 1024x1024x128 cells at 2 m, 8x8 cuboid buildings,
-water vapour + TKE + 2 passive tracers, WENO-9, TKE closure, FP64.  One step = one
-Dynamics_Euler_Stratified_WenoFV::time_step (3 stages) + one LES_Closure::apply, inputs resident in
-HBM.  Timed with HIP events on the library's stream.
+Workload (default): neutral urban ABL, 1024x1024x128 cells at 2 m, 8x8 cuboid buildings, rough
+wall (log-law surface fluxes, z0 = 0.1 m), water vapour + TKE + 2 passive tracers, WENO-9, TKE
+closure, FP64.  One step = one Dynamics_Euler_Stratified_WenoFV::time_step (3 stages) + one
+apply_surface_fluxes + one LES_Closure::apply, inputs resident in HBM.  Timed with HIP events on
+the library's stream.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2]
                     [--config abl_fp64|abl_fp32|abl_fp32_learned|bubble]
@@ -58,11 +59,15 @@ def cpu_baseline(cfg_full, les, seconds_budget=25.0):
     od = oracle_py.Dycore(cfg, st, imm, nthreads=cores)
     dt = od.compute_time_step()
     n3 = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    shape = (cfg["nz"], cfg["ny"], cfg["nx"])
+    rough, zero3 = np.full(shape, 0.1), np.zeros(shape)
+    srough, zero2 = np.full(shape[1:], 0.1), np.zeros(shape[1:])
     steps, t_tot = 0, 0.0
     while True:
         t0 = time.perf_counter()
         od.time_step(st, dt)
         if les:
+            oracle_py.surface_fluxes(cfg, st, imm, rough, zero3, srough, zero2, dt)
             oracle_py.les_apply(cfg, st, imm, dt, nthreads=cores)
         t_tot += time.perf_counter() - t0
         steps += 1
@@ -121,7 +126,7 @@ def main():
     for _ in range(2):
         step()
     d.synchronize()
-    fams = ["weno_x", "weno_y", "weno_z", "prep", "les", "sgs"]
+    fams = ["weno_x", "weno_y", "weno_z", "prep", "surface", "les", "sgs"]
     times = {f: d.kernel_time(f) for f in fams}
     d.kernel_timing(False)
     dom = max(["weno_x", "weno_y", "weno_z"], key=lambda f: times[f][0])
@@ -138,12 +143,11 @@ def main():
     # recorded under profiles/ for this workload (null for workloads not profiled)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_%s.json" % cfg["name"].replace("neutral_urban_abl", "abl_urban")
-                         .replace("_weno9_fp64", "_fp64"))
-    if os.path.exists(tfile):
-        tj = json.load(open(tfile))
-        key = "k_sweep<%d, %s, %d>" % (cfg["ord"], "double" if fp64 else "float", "xyz".index(dom[-1]))
-        if key in tj["bytes_per_launch"]:
-            traffic = tj["bytes_per_launch"][key]["read"] + tj["bytes_per_launch"][key]["write"]
+                         .replace("_weno9_", "_"))
+    if os.path.exists(tfile) and not learned:
+        fam = json.load(open(tfile)).get("families", {}).get(dom)
+        if fam:
+            traffic = fam["read"] + fam["write"]
     roof = {"bound": "valu", "kernel": "k_sweep<%d,%s,%s>" % (cfg["ord"], "double" if fp64 else "float", dom[-1]),
             "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic,
@@ -157,7 +161,7 @@ def main():
                       "path": "C++ Coupler + modules",
                       "closure": ("learned (bf16 matrix cores) + tke fallback" if learned else "tke") if les else "none",
                       "tracers": cfg["num_tracers"],
-                      "step": "dycore.time_step + les_closure.apply" if les else "dycore.time_step"},
+                      "step": "dycore.time_step + apply_surface_fluxes + les_closure.apply" if les else "dycore.time_step"},
            "roofline": roof}
     d.destroy()
     if not args.no_cpu_baseline:

@@ -113,6 +113,16 @@ int  pu_les_closure_init(pu_ctx *ctx, const pu_coupler_state *cs);
 /* LES_Closure::apply (les_closure.h:30-323).  In place on the coupler state; requires idTKE. */
 int  pu_les_closure_apply(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys);
 
+/* modules::apply_surface_fluxes (model/modules/surface_flux.h:9-126): log-law drag (kappa = 0.4)
+ * and temperature exchange against every immersed neighbour, forward Euler on uvel, vvel, wvel,
+ * temp.  _init builds the halo'd roughness/temperature boxes the way custom_modules::sc_init builds
+ * immersed_{roughness,temp}_halos (sc_init.h:728-767): immersed_roughness/immersed_temp
+ * [nz][ny][nx] and surface_roughness/surface_temp [ny][nx] (device), x/y periodic, the bottom halo
+ * from the surface fields, the top halo 0.  Needs pu_dycore_init or pu_les_closure_init first. */
+int  pu_surface_fluxes_init(pu_ctx *ctx, const double *immersed_roughness, const double *immersed_temp,
+                            const double *surface_roughness, const double *surface_temp);
+int  pu_surface_fluxes_apply(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys);
+
 /* FP64 reductions on the device: total mass sum(rho)*dV (rho = dry + mass-adding tracers), max
  * wind speed and max |w| (the quantities of Coupler::inform_user, coupler.h:507-530). */
 int  pu_diagnostics(pu_ctx *ctx, const pu_coupler_state *cs, double *mass, double *max_wind,

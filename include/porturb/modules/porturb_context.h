@@ -20,6 +20,7 @@ struct Context {
   bool dycore_ready = false, les_ready = false;
   pu_mlp *sgs = nullptr;      // learned subgrid closure (option les_sgs_model), if any
   bool sgs_checked = false;
+  bool sfc_ready = false;     // surface-flux roughness/temperature boxes built
   ~Context() {
     if (ctx) pu_destroy(ctx);
     if (sgs) pu_mlp_destroy(sgs);

@@ -127,6 +127,11 @@ void   oc_sgs_infer(const oc_sgs_model *M, const double *les, const double *imm,
 int    oc_les_apply_learned(const oc_config *cfg, double *coupler, const double *immersed_prop, double dt_phys,
                             const oc_sgs_model *M, int bf16, float *tau_out, unsigned char *flags_out);
 
+/* modules::apply_surface_fluxes (surface_flux.h:9-126), in place on the coupler state; prop,
+ * rough, temp [nz][ny][nx], sfc_rough, sfc_temp [ny][nx]. */
+int    oc_surface_fluxes(const oc_config *c, double *coupler, const double *prop, const double *rough,
+                         const double *temp, const double *sfc_rough, const double *sfc_temp, double dt);
+
 /* Sum over cells of the dycore's full density rho (dry + mass-adding tracers), times cell volume. */
 double oc_total_mass(const oc_config *cfg, const double *coupler);
 

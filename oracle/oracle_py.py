@@ -97,6 +97,8 @@ def lib():
         L.oc_sgs_infer.restype = None
         L.oc_les_apply_learned.argtypes = [cp, P, P, D, C.POINTER(OcSgsModel), I, P, P]
         L.oc_les_apply_learned.restype = I
+        L.oc_surface_fluxes.argtypes = [cp, P, P, P, P, P, P, D]
+        L.oc_surface_fluxes.restype = I
         _lib = L
     return _lib
 
@@ -297,3 +299,11 @@ def les_apply_learned(cfg, coupler, imm, dt, model, bf16=True, nthreads=0):
     assert lib().oc_les_apply_learned(C.byref(oc), coupler.ctypes.data, imm.ctypes.data, dt, C.byref(model.s),
                                       int(bf16), tau.ctypes.data, flags.ctypes.data) == 0
     return tau, flags
+
+
+def surface_fluxes(cfg, coupler, prop, rough, temp, sfc_rough, sfc_temp, dt):
+    """modules::apply_surface_fluxes in place on the coupler state (float64 arrays)."""
+    oc = to_oc_config(cfg)
+    assert coupler.flags["C_CONTIGUOUS"] and coupler.dtype == np.float64
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (prop, rough, temp, sfc_rough, sfc_temp)]
+    assert lib().oc_surface_fluxes(C.byref(oc), coupler.ctypes.data, *[a.ctypes.data for a in arrs], dt) == 0

@@ -114,6 +114,8 @@ def lib():
         "pu_sgs_features": (I, [P, I, I, I, P, P, C.c_longlong, P]),
         "pu_les_closure_set_model": (I, [P, P]),
         "pu_les_closure_sgs": (I, [P, csp, D, P, P]),
+        "pu_surface_fluxes_init": (I, [P, P, P, P, P]),
+        "pu_surface_fluxes_apply": (I, [P, csp, D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -296,6 +298,22 @@ class Dycore:
         tau = np.zeros(shape, np.int32)
         check(lib().pu_dycore_immersed_masks(self.ctx, any10.ctypes.data, tau.ctypes.data))
         return any10, tau
+
+    def surface_init(self, rough, temp, sfc_rough, sfc_temp):
+        """Host arrays immersed_roughness, immersed_temp [nz][ny][nx], surface_roughness,
+        surface_temp [ny][nx]; uploaded and turned into the halo'd boxes."""
+        bufs = []
+        for a in (rough, temp, sfc_rough, sfc_temp):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            b = DeviceBuffer(a.nbytes)
+            b.upload(a)
+            bufs.append(b)
+        check(lib().pu_surface_fluxes_init(self.ctx, *[b.ptr for b in bufs]))
+        for b in bufs:
+            b.free()
+
+    def surface_apply(self, cs, dt):
+        check(lib().pu_surface_fluxes_apply(self.ctx, C.byref(cs.c), dt))
 
     def set_sgs_model(self, mlp):
         check(lib().pu_les_closure_set_model(self.ctx, mlp.h if mlp is not None else None))

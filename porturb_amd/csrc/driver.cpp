@@ -3,8 +3,9 @@
 // Mirrors the structure of the reference's ABL drivers (experiments/examples/abl_neutral.cpp:
 // 65-179): set options and constants as custom_modules::sc_init does (sc_init.h:31-51), allocate
 // the coupled state, add tracers, initialise the case, init the LES closure and the dycore, then
-// per step run_module(dycore.time_step) and run_module(les_closure.apply) with
-// dt = dycore.compute_time_step.  Exposed through pu_driver_* for bench.py and the tests.
+// per step run_module(dycore.time_step), run_module(apply_surface_fluxes) for the rough-wall ABL
+// cases and run_module(les_closure.apply) with dt = dycore.compute_time_step (the reference's
+// module order, abl_neutral.cpp:98-103, without its geostrophic forcing, sponge and averager).  Exposed through pu_driver_* for bench.py and the tests.
 #include <cmath>
 #include <memory>
 #include <string>
@@ -13,12 +14,14 @@
 #include "porturb/core/coupler.h"
 #include "porturb/modules/dynamics_rk.h"
 #include "porturb/modules/les_closure.h"
+#include "porturb/modules/surface_flux.h"
 
 struct pu_driver {
   core::Coupler coupler;
   modules::Dynamics_Euler_Stratified_WenoFV dycore;
   modules::LES_Closure les;
   bool closure = true;
+  bool surface = false;     // rough-wall surface fluxes (the ABL cases)
   std::string error;
 };
 
@@ -53,6 +56,23 @@ int pu_driver_create(int case_id, int nx, int ny, int nz, double dx, int ord, in
     auto &dm = c.get_data_manager_readwrite();
     for (const char *name : {"density_dry", "uvel", "vvel", "wvel", "temp", "immersed_proportion"})
       dm.register_and_allocate<double>(name, "", {nz, ny, nx}, {"z", "y", "x"});
+    if (case_id != PU_CASE_BUBBLE) {
+      // rough wall: sc_init's roughness entries (sc_init.h:52,63-88), roughness 0.1 m, no wall heating
+      d->surface = true;
+      c.set_option<double>("roughness", 0.1);
+      dm.register_and_allocate<double>("immersed_roughness", "", {nz, ny, nx}, {"z", "y", "x"});
+      dm.register_and_allocate<double>("immersed_temp", "", {nz, ny, nx}, {"z", "y", "x"});
+      dm.register_and_allocate<double>("surface_roughness", "", {ny, nx}, {"y", "x"});
+      dm.register_and_allocate<double>("surface_temp", "", {ny, nx}, {"y", "x"});
+      auto fill = [&](const char *name, double v) {
+        auto e = dm.get_collapsed<double>(name);
+        if (pu_fill_device(e.data(), v, (size_t)e.size()) != PU_OK) core::endrun(pu_last_error());
+      };
+      fill("immersed_roughness", 0.1);
+      fill("immersed_temp", 0.);
+      fill("surface_roughness", 0.1);
+      fill("surface_temp", 0.);
+    }
     c.add_tracer("water_vapor", "water_vapor", true, true, true);
     c.set_option<int>("idWV", c.get_tracer_index("water_vapor"));
     if (d->closure) d->les.init(c);                  // adds TKE (= 0.1)
@@ -81,6 +101,7 @@ int pu_driver_step(pu_driver *d, int nsteps) {
     for (int n = 0; n < nsteps; n++) {
       double dt = d->dycore.compute_time_step(c);
       c.run_module([&](core::Coupler &cc) { d->dycore.time_step(cc, dt); }, "dycore");
+      if (d->surface) c.run_module([&](core::Coupler &cc) { modules::apply_surface_fluxes(cc, dt); }, "surface_fluxes");
       if (d->closure) c.run_module([&](core::Coupler &cc) { d->les.apply(cc, dt); }, "les_closure");
       c.set_option<double>("elapsed_time", c.get_option<double>("elapsed_time") + dt);
     }

@@ -7,9 +7,10 @@
 //   sweep x/y/z: WENO reconstruction of every field for every cell of a line segment, the
 //               acoustic upwind Riemann flux at each face and the flux divergence, with the face
 //               states exchanged through LDS -- no edge arrays in HBM (:572-728, :808-912);
-//   the z sweep, as it finishes each cell's tendencies, also applies the gravity/Coriolis sources,
-//   the SSPRK stage combination, positive-tracer clipping and the immersed-boundary relaxation
-//   (:308-362, :370-473), so the finished tendencies never return to HBM.
+//   the sweeps run z, y, x; the x sweep, as it finishes each cell's tendencies, also applies the
+//   gravity/Coriolis sources, the SSPRK stage combination, positive-tracer clipping and the
+//   immersed-boundary relaxation (:308-362, :370-473), so the finished tendencies never return to
+//   HBM.
 // Layout: every field SoA [var][z][y][x], x fastest.  The state is double (PU_FP64) or float with a
 // float compensation term for rho and rho*theta (PU_FP32).
 #include "pu_common.h"
@@ -109,19 +110,6 @@ __global__ __launch_bounds__(256) void k_prep(const ST *__restrict__ U, const ST
 // ------------------------------------------------------------------------------------------------
 // WENO limiter (WenoLimiter.h compute_limited_edges / gll_1 / gll_2 / convexify / map_rs)
 // ------------------------------------------------------------------------------------------------
-// Reciprocal of a positive, normal argument: v_rcp_f32 in FP32; in FP64 v_rcp_f64 refined by two
-// Newton steps (about 1 ulp; every call site divides by a sum of positive terms >= 1e-20).
-template <class T> __device__ __forceinline__ T recip(T x) {
-  if constexpr (std::is_same<T, float>::value) {
-    return __builtin_amdgcn_rcpf(x);
-  } else {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-  }
-}
 
 template <class T, int S> __device__ __forceinline__ void convexify(T *w) {
   T tot = w[0];
@@ -375,9 +363,11 @@ __device__ __forceinline__ Upw<T> riemann(T rL, T uL, T pL, T rR, T uR, T pR, T 
   return u;
 }
 
-// FUSE (z sweep only): each cell's final tendency goes straight into the stage combination
-// (combine_field) instead of back to `tend`.
-template <int ORD, class T, int DIR, bool FUSE>
+// MODE: 0 = first sweep of the stage (writes the tendencies), 1 = adds to them, 2 = last sweep:
+// each cell's final tendency goes straight into the stage combination (combine_field) instead of
+// back to `tend`.  The stage runs z (0), y (1), x (2): the last sweep is the one whose rows are
+// 64 cells wide, so the stage-state reads and writes of the combination are coalesced.
+template <int ORD, class T, int DIR, int MODE>
 __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep(const T *__restrict__ prim, const uint8_t *__restrict__ immf,
                                                const uint8_t *__restrict__ any10, T *__restrict__ tend,
                                                SweepParams P, FuseArgs<T> FA) {
@@ -459,17 +449,18 @@ __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep
   const long long cid = ((long long)k * P.ny + j) * P.nx + i;
   const T rd = (T)P.rd;
   double f_rho = 0, f_u = 0, f_v = 0, f_mult = 0, f_trr = 0, f_trt = 0;
-  if constexpr (FUSE) {
+  if constexpr (MODE == 2) {
     if (has_tend) stage_inputs(FA, cid, k, f_rho, f_u, f_v, f_mult, f_trr, f_trt);
   }
   auto accumulate = [&](int fidx, T v) {
     T *tp = tend + (long long)fidx * P.n3 + cid;
-    if constexpr (FUSE) {
+    if constexpr (MODE == 2) {
       const T tsum = *tp + v;
       combine_field(FA, fidx, cid, (double)tsum, f_rho, f_u, f_v, f_mult, f_trr, f_trt);
+    } else if constexpr (MODE == 1) {
+      *tp = *tp + v;
     } else {
-      if (DIR == 0) *tp = v;
-      else *tp = *tp + v;
+      *tp = v;
     }
   };
 
@@ -559,10 +550,16 @@ __global__ __launch_bounds__(256) void k_c2d(const double *__restrict__ rd_, con
   if (id >= P.n3) return;
   int k = (int)(id / ((long long)P.nx * P.ny));
   double rho_d = rd_[id], u = uu[id], v = vv[id], w = ww[id], temp = tt[id];
+  // all tracer loads before the first store (the stores may alias them, so a load-store loop would
+  // wait on every load)
+  double m[PU_MAX_TRACERS];
+#pragma unroll
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++) m[tr] = tr < P.ntr ? trc.p[tr][id] : 0.;
   double rho_v = trc.p[P.idWV][id];
   double press = rho_d * P.R_d * temp + rho_v * P.R_v * temp;
   double rho = rho_d;
-  for (int tr = 0; tr < P.ntr; tr++) if ((P.adds_mass_mask >> tr) & 1u) rho += trc.p[tr][id];
+#pragma unroll
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++) if (tr < P.ntr && ((P.adds_mass_mask >> tr) & 1u)) rho += m[tr];
   double theta = pow(press / P.C0, 1. / P.gamma) / rho;
   double vals[NUM_STATE] = {rho, rho * u, rho * v, rho * w, rho * theta};
   double mult = P.relax ? P.relax_mult[id] : 0.;
@@ -575,8 +572,10 @@ __global__ __launch_bounds__(256) void k_c2d(const double *__restrict__ rd_, con
     }
     store_cons(U, Ulo, l, id, P.n3, x);
   }
-  for (int tr = 0; tr < P.ntr; tr++) {
-    double x = trc.p[tr][id];
+#pragma unroll
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++) {
+    if (tr >= P.ntr) break;
+    double x = m[tr];
     if (P.relax) {
       x = x + (0. - x) * mult;
       if ((P.positive_mask >> tr) & 1u) x = fmax(0., x);
@@ -596,14 +595,21 @@ __global__ __launch_bounds__(256) void k_d2c(const ST *__restrict__ U, const ST 
   double v = load_cons(U, Ulo, idV, id, P.n3) / rho;
   double w = load_cons(U, Ulo, idW, id, P.n3) / rho;
   double theta = load_cons(U, Ulo, idT, id, P.n3) / rho;
+  // all tracer loads before the first store
+  double m[PU_MAX_TRACERS];
+#pragma unroll
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++) m[tr] = tr < P.ntr ? load_cons(U, Ulo, NUM_STATE + tr, id, P.n3) : 0.;
   double press = P.C0 * pow(rho * theta, P.gamma);
   double rho_v = load_cons(U, Ulo, NUM_STATE + P.idWV, id, P.n3);
   double rho_d = rho;
-  for (int tr = 0; tr < P.ntr; tr++)
-    if ((P.adds_mass_mask >> tr) & 1u) rho_d -= load_cons(U, Ulo, NUM_STATE + tr, id, P.n3);
+#pragma unroll
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++)
+    if (tr < P.ntr && ((P.adds_mass_mask >> tr) & 1u)) rho_d -= m[tr];
   double temp = press / (rho_d * P.R_d + rho_v * P.R_v);
   rd_[id] = rho_d; uu[id] = u; vv[id] = v; ww[id] = w; tt[id] = temp;
-  for (int tr = 0; tr < P.ntr; tr++) trc.p[tr][id] = load_cons(U, Ulo, NUM_STATE + tr, id, P.n3);
+#pragma unroll
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++)
+    if (tr < P.ntr) trc.p[tr][id] = m[tr];
 }
 
 template <class ST>
@@ -653,9 +659,9 @@ int launch_prep(pu_ctx *ctx, const void *U, const void *Ulo, hipStream_t s) {
 
 static CombineParams combine_params(pu_ctx *ctx);
 
-// stage < 0: tendencies only (into ctx->d_tend); otherwise the z sweep also combines stage `stage`
+// mode: 0 first sweep (writes d_tend), 1 adds to d_tend, 2 adds and combines stage `stage`
 template <int ORD, class T, int DIR>
-static int sweep_one(pu_ctx *ctx, hipStream_t s, int stage, double dt, int nrelax) {
+static int sweep_one(pu_ctx *ctx, hipStream_t s, int mode, int stage, double dt, int nrelax) {
   const pu::Grid &g = ctx->g;
   SweepParams P;
   P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ntr = ctx->cfg.num_tracers;
@@ -676,26 +682,29 @@ static int sweep_one(pu_ctx *ctx, hipStream_t s, int stage, double dt, int nrela
   else if (DIR == 1) grd = dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)((g.ny + NC - 1) / NC), (unsigned)g.nz);
   else grd = dim3((unsigned)((g.nx + BX - 1) / BX), (unsigned)g.ny, (unsigned)((g.nz + NC - 1) / NC));
   FuseArgs<T> FA{};
-  if (DIR == 2 && stage >= 0) {
+  const T *prim = (const T *)ctx->d_prim;
+  T *tend = (T *)ctx->d_tend;
+  if (mode == 2) {
     FA.U = (T *)ctx->d_U; FA.Ulo = (T *)ctx->d_Ulo; FA.Us = (T *)ctx->d_Us; FA.Uslo = (T *)ctx->d_Uslo;
     FA.C = combine_params(ctx);
     FA.C.stage = stage; FA.C.dt = dt; FA.C.nrelax = nrelax;
-    hipLaunchKernelGGL((k_sweep<ORD, T, DIR, true>), grd, blk, 0, s, (const T *)ctx->d_prim, ctx->d_imm_flag,
-                       ctx->d_any10, (T *)ctx->d_tend, P, FA);
+    hipLaunchKernelGGL((k_sweep<ORD, T, DIR, 2>), grd, blk, 0, s, prim, ctx->d_imm_flag, ctx->d_any10, tend, P, FA);
+  } else if (mode == 1) {
+    hipLaunchKernelGGL((k_sweep<ORD, T, DIR, 1>), grd, blk, 0, s, prim, ctx->d_imm_flag, ctx->d_any10, tend, P, FA);
   } else {
-    hipLaunchKernelGGL((k_sweep<ORD, T, DIR, false>), grd, blk, 0, s, (const T *)ctx->d_prim, ctx->d_imm_flag,
-                       ctx->d_any10, (T *)ctx->d_tend, P, FA);
+    hipLaunchKernelGGL((k_sweep<ORD, T, DIR, 0>), grd, blk, 0, s, prim, ctx->d_imm_flag, ctx->d_any10, tend, P, FA);
   }
   PU_HIP_CHECK(hipGetLastError());
   return PU_OK;
 }
 
+// z, then y, then x (the x sweep finishes the tendencies and, for stage >= 0, combines the stage)
 template <int ORD, class T>
 static int sweeps_t(pu_ctx *ctx, hipStream_t s, int stage, double dt, int nrelax) {
   int rc;
-  if ((rc = sweep_one<ORD, T, 0>(ctx, s, stage, dt, nrelax))) return rc;
-  if (ctx->g.ny > 1 && (rc = sweep_one<ORD, T, 1>(ctx, s, stage, dt, nrelax))) return rc;
-  if ((rc = sweep_one<ORD, T, 2>(ctx, s, stage, dt, nrelax))) return rc;
+  if ((rc = sweep_one<ORD, T, 2>(ctx, s, 0, stage, dt, nrelax))) return rc;
+  if (ctx->g.ny > 1 && (rc = sweep_one<ORD, T, 1>(ctx, s, 1, stage, dt, nrelax))) return rc;
+  if ((rc = sweep_one<ORD, T, 0>(ctx, s, stage >= 0 ? 2 : 1, stage, dt, nrelax))) return rc;
   return PU_OK;
 }
 

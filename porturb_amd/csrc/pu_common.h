@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstddef>
+#include <type_traits>
 #include <string>
 #include <map>
 #include <vector>
@@ -69,6 +70,9 @@ struct pu_ctx {
   uint8_t *d_sgs_valid = nullptr;              // [n3]
   float   *d_sgs_fmul = nullptr;               // [108]
   bool     sgs_valid_ready = false;
+  // surface fluxes: halo'd immersed roughness and temperature boxes
+  double  *d_sfc_rough = nullptr, *d_sfc_temp = nullptr;
+  bool     sfc_ready = false;
   // reductions
   double *d_red = nullptr;                     // scratch for reductions
   // timing
@@ -110,6 +114,20 @@ __device__ __forceinline__ void swizzled_block(unsigned &bx, unsigned &by, unsig
   bz = l / (gridDim.x * gridDim.y);
 }
 
+// Reciprocal of a positive, normal argument: v_rcp_f32 in FP32; in FP64 v_rcp_f64 refined by two
+// Newton steps (about 1 ulp; every call site divides by a positive quantity >= 1e-20).
+template <class T> __device__ __forceinline__ T recip(T x) {
+  if constexpr (std::is_same<T, float>::value) {
+    return __builtin_amdgcn_rcpf(x);
+  } else {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+  }
+}
+
 // Tracer pointer bundles passed by value to kernels.
 struct TracerPtrs { double *p[PU_MAX_TRACERS]; };
 struct TracerPtrsC { const double *p[PU_MAX_TRACERS]; };
@@ -135,8 +153,9 @@ void collect_timers(pu_ctx *ctx);
 
 // Launchers implemented in the kernel translation units.
 int launch_prep(pu_ctx *ctx, const void *U, const void *Ulo, hipStream_t s);
-// stage < 0: tendencies into d_tend; stage 0..2: the z sweep also applies the stage combination
-// (sources, SSPRK3 stage, clip, nrelax immersed relaxations) for every cell it finishes
+// Sweeps z, y, x.  stage < 0: tendencies into d_tend; stage 0..2: the x sweep also applies the
+// stage combination (sources, SSPRK3 stage, clip, nrelax immersed relaxations) for every cell it
+// finishes
 int launch_sweeps(pu_ctx *ctx, hipStream_t s, int stage = -1, double dt = 0., int nrelax = 0);
 int launch_convert_c2d(pu_ctx *ctx, const pu_coupler_state *cs, int relax, hipStream_t s);
 int launch_convert_d2c(pu_ctx *ctx, pu_coupler_state *cs, hipStream_t s);

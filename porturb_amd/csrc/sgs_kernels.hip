@@ -46,9 +46,30 @@ __device__ __forceinline__ fx4 mfma16(bfx4 a, bfx4 b, fx4 c) {
 
 __device__ __forceinline__ bfx4 ld4(const short *p) { return *reinterpret_cast<const bfx4 *>(p); }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
-__device__ __forceinline__ float gelu_deriv(float x) {
-  return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.39894228040143268f;
+// GELU(x) = x Phi(x), Phi(x) = (1 + erf(x / sqrt 2)) / 2, with erf from Abramowitz & Stegun,
+// Handbook of Mathematical Functions, formula 7.1.26 (|error| <= 1.5e-7, far below the BF16
+// rounding of the activations): for z >= 0, erf(z) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5))))
+// exp(-z^2), t = 1 / (1 + p z).  Branch-free: e = 1 - erf(|x| / sqrt 2) is formed once and
+// Phi = 1 - e/2 for x >= 0, e/2 for x < 0.
+__device__ __forceinline__ float phi_tail(float x, float &ex) {
+  const float z = fabsf(x) * 0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.f));
+  const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
+                             0.254829592f);
+  ex = __builtin_amdgcn_exp2f(-z * z * 1.44269504088896341f);           // exp(-x^2 / 2)
+  return poly * ex;                                                     // 1 - erf(|x| / sqrt 2)
+}
+__device__ __forceinline__ float gelu(float x) {
+  float ex;
+  const float e = phi_tail(x, ex);
+  const float phi = x >= 0.f ? fmaf(-0.5f, e, 1.f) : 0.5f * e;
+  return x * phi;
+}
+__device__ __forceinline__ float gelu_deriv(float x) {                   // Phi(x) + x phi(x)
+  float ex;
+  const float e = phi_tail(x, ex);
+  const float phi = x >= 0.f ? fmaf(-0.5f, e, 1.f) : 0.5f * e;
+  return fmaf(x * 0.39894228040143268f, ex, phi);
 }
 
 // sum over the 16 lanes that share l/16 (one column group of a D fragment)
@@ -104,16 +125,24 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs A) {
   __shared__ float sLoss[4];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, r16 = lane & 15, g = lane >> 4;
   const long long base = (long long)blockIdx.x * SAMPLES_PER_BLOCK;
-  // stage the block's inputs (rows beyond n are zero)
-  for (int c = tid; c < SAMPLES_PER_BLOCK * (NINP / 4); c += 256) {
+  // stage the block's inputs (rows beyond n are zero); all loads issued before the LDS writes
+  constexpr int PER = SAMPLES_PER_BLOCK * (NINP / 4) / 256;
+  uint2 xin[PER];
+#pragma unroll
+  for (int e = 0; e < PER; e++) {
+    const int c = tid + 256 * e;
     const int s = c / (NINP / 4), q = c % (NINP / 4);
     const long long gs = base + s;
-    uint2 v = make_uint2(0u, 0u);
+    xin[e] = make_uint2(0u, 0u);
     if (gs < A.n) {
       const long long row = A.idx ? (long long)A.idx[gs] : gs;
-      v = *reinterpret_cast<const uint2 *>(A.X + row * NINP + 4 * q);
+      xin[e] = *reinterpret_cast<const uint2 *>(A.X + row * NINP + 4 * q);
     }
-    *reinterpret_cast<uint2 *>(sX + s * NINP + 4 * q) = v;
+  }
+#pragma unroll
+  for (int e = 0; e < PER; e++) {
+    const int c = tid + 256 * e;
+    *reinterpret_cast<uint2 *>(sX + (c / (NINP / 4)) * NINP + 4 * (c % (NINP / 4))) = xin[e];
   }
   __syncthreads();
   const int my = 16 * w + r16;                    // this lane's sample column within the block
@@ -350,10 +379,12 @@ __global__ __launch_bounds__(256) void k_sgs_filter_halo(FilterArgs A) {
   }
 }
 
-// One input of a cell: neighbour n of the 3x3x3 box, quantity q (u, v, w, theta).
-__device__ __forceinline__ float sgs_input(const double *__restrict__ F, long long nh, long long c, long long nb, int q,
+// One input of a cell: neighbour n of the 3x3x3 box, quantity q (u, v, w, theta); the difference
+// is formed in the box's precision, rounded to float once.
+template <class T>
+__device__ __forceinline__ float sgs_input(const T *__restrict__ F, long long nh, long long c, long long nb, int q,
                                            float fm) {
-  const double *Fq = F + (long long)(1 + q) * nh;
+  const T *Fq = F + (long long)(1 + q) * nh;
   return (float)(Fq[nb] - Fq[c]) * fm;
 }
 
@@ -403,8 +434,8 @@ __global__ __launch_bounds__(256) void k_sgs_valid(const double *__restrict__ im
   valid[id] = ok ? 1 : 0;
 }
 
-struct InferArgs {
-  const double *F;
+template <class T> struct InferArgs {
+  const T *F;
   long long nxh, nyh, nh, n3;
   int nx, ny, nz;
   const unsigned char *valid;
@@ -418,13 +449,21 @@ struct InferArgs {
   unsigned char *flags;
 };
 
-__global__ __launch_bounds__(256) void k_sgs_infer(InferArgs A) {
+// One group = 64 consecutive cells of an x row (i0..i0+63 at j, k); a block loops over groups.
+// The group's 3x3 neighbourhood rows of (u, v, w, theta), 66 x 3 x 3 points, are staged in LDS
+// with row-contiguous loads; each wave then takes 16 cells: inputs from LDS, the network on the
+// matrix cores, and (lanes 0-15) the outputs and the fallback rule.
+constexpr int GX = 64, GBX = GX + 2, GBN = GBX * 3 * 3;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_sgs_infer(InferArgs<T> A) {
   __shared__ __align__(16) short sW1[NH * NINP];
   __shared__ __align__(16) short sW2[NH * NH];
   __shared__ __align__(16) short sW3[NOUTP * NH];
   __shared__ float sB[NH + NH + NOUTP];
   __shared__ float sFm[NINP];
   __shared__ float sY[4][16][NOUTP + 1];
+  __shared__ T sQ[4 * GBN];                     // u, v, w, theta of the group's neighbourhood
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, r16 = lane & 15, g = lane >> 4;
   for (int p = tid; p < NH * NINP / 4; p += 256)
     reinterpret_cast<uint2 *>(sW1)[p] = reinterpret_cast<const uint2 *>(A.wbf + P_W1)[p];
@@ -435,16 +474,32 @@ __global__ __launch_bounds__(256) void k_sgs_infer(InferArgs A) {
   for (int p = tid; p < NH + NH + NOUTP; p += 256)
     sB[p] = p < NH ? A.master[P_B1 + p] : (p < 2 * NH ? A.master[P_B2 + p - NH] : A.master[P_B3 + p - 2 * NH]);
   for (int p = tid; p < NINP; p += 256) sFm[p] = p < NIN ? A.fmul[p] : 0.f;
-  __syncthreads();
-  const long long sz = A.nxh * A.nyh, sy = A.nxh;
-  const long long ngroups = (A.n3 + 63) / 64;
+  const long long ngx = (A.nx + GX - 1) / GX;
+  const long long ngroups = ngx * A.ny * A.nz;
   for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    const long long cell = grp * 64 + 16 * w + r16;
-    const bool in = cell < A.n3;
-    const long long id = in ? cell : A.n3 - 1;
-    const int i = (int)(id % A.nx), j = (int)((id / A.nx) % A.ny), k = (int)(id / ((long long)A.nx * A.ny));
-    const long long c = ((long long)(k + 1) * A.nyh + (j + 1)) * A.nxh + (i + 1);
-    // inputs: lane group g takes neighbours 4*kb + g
+    const int i0 = (int)(grp % ngx) * GX, j = (int)((grp / ngx) % A.ny), k = (int)(grp / (ngx * A.ny));
+    // all loads of the staging issued before any LDS write (one wait, not one per element)
+    constexpr int PER = (4 * GBN + 255) / 256;
+    T buf[PER];
+#pragma unroll
+    for (int e = 0; e < PER; e++) {
+      const int p = tid + 256 * e;
+      const int q = p / GBN, r = p % GBN;
+      const int lx = r % GBX, ly = (r / GBX) % 3, lz = r / (3 * GBX);
+      const long long ih = i0 + lx, jh = j + ly, kh = k + lz;       // closure-box coordinates
+      buf[e] = (p < 4 * GBN && ih < A.nxh) ? A.F[(long long)(1 + q) * A.nh + (kh * A.nyh + jh) * A.nxh + ih] : T(0);
+    }
+    __syncthreads();                            // the previous group is done with sQ and sY
+#pragma unroll
+    for (int e = 0; e < PER; e++) {
+      const int p = tid + 256 * e;
+      if (p < 4 * GBN) sQ[p] = buf[e];
+    }
+    __syncthreads();
+    const int xl = 16 * w + r16;                // this lane's cell within the group
+    const int i = i0 + xl;
+    const bool in = i < A.nx;
+    const int c = (1 * 3 + 1) * GBX + xl + 1;  // the cell's index in the staged box
     bfx4 xb[NINP / 16];
     float mx = 0.f;
     for (int kb = 0; kb < NINP / 16; kb++) {
@@ -452,9 +507,9 @@ __global__ __launch_bounds__(256) void k_sgs_infer(InferArgs A) {
       bfx4 v = {0, 0, 0, 0};
       if (n < 27) {
         const int dz = n / 9 - 1, dy = (n / 3) % 3 - 1, dx = n % 3 - 1;
-        const long long nb = c + dz * sz + dy * sy + dx;
+        const int nb = c + (dz * 3 + dy) * GBX + dx;
         for (int q = 0; q < 4; q++) {
-          const float x = sgs_input(A.F, A.nh, c, nb, q, sFm[4 * n + q]);
+          const float x = (float)(sQ[q * GBN + nb] - sQ[q * GBN + c]) * sFm[4 * n + q];
           mx = fmaxf(mx, fabsf(x));
           v[q] = bf16_bits(x);
         }
@@ -480,6 +535,7 @@ __global__ __launch_bounds__(256) void k_sgs_infer(InferArgs A) {
     for (int r = 0; r < 4; r++) sY[w][r16][4 * g + r] = y[r] + sB[2 * NH + 4 * g + r];
     __syncthreads();
     if (g == 0 && in) {
+      const long long id = ((long long)k * A.ny + j) * A.nx + i;
       float t[NOUT];
       for (int o = 0; o < NOUT; o++) {
         t[o] = __fadd_rn(A.out_mean[o], __fmul_rn(A.out_scale[o], sY[w][r16][o]));
@@ -487,7 +543,8 @@ __global__ __launch_bounds__(256) void k_sgs_infer(InferArgs A) {
       }
       unsigned char fl = 0;
       if (A.valid[id] && mx <= A.xmax) {
-        const double *U = A.F + A.nh, *V = A.F + 2 * A.nh, *W = A.F + 3 * A.nh, *TH = A.F + 4 * A.nh;
+        const T *U = sQ, *V = sQ + GBN, *W = sQ + 2 * GBN, *TH = sQ + 3 * GBN;
+        const int sy = GBX, sz = 3 * GBX;
         const double ux = (U[c + 1] - U[c - 1]) * A.r2dx, uy = (U[c + sy] - U[c - sy]) * A.r2dy, uz = (U[c + sz] - U[c - sz]) * A.r2dz;
         const double vx = (V[c + 1] - V[c - 1]) * A.r2dx, vy = (V[c + sy] - V[c - sy]) * A.r2dy, vz = (V[c + sz] - V[c - sz]) * A.r2dz;
         const double wx = (W[c + 1] - W[c - 1]) * A.r2dx, wy = (W[c + sy] - W[c - sy]) * A.r2dy, wz = (W[c + sz] - W[c - sz]) * A.r2dz;
@@ -502,7 +559,6 @@ __global__ __launch_bounds__(256) void k_sgs_infer(InferArgs A) {
       }
       A.flags[id] = fl;
     }
-    __syncthreads();
   }
 }
 
@@ -574,7 +630,8 @@ static int mlp_reduce(pu_mlp *m, int nblocks, const float *gin, float *grad_out,
   return PU_OK;
 }
 
-int sgs_infer_device(pu_ctx *ctx, double dt) {
+template <class T>
+static int sgs_infer_t(pu_ctx *ctx, double dt) {
   const pu_config &c = ctx->cfg;
   const pu_mlp *m = ctx->sgs_model;
   const long long n3 = (long long)c.nx * c.ny * c.nz;
@@ -585,8 +642,8 @@ int sgs_infer_device(pu_ctx *ctx, double dt) {
     ctx->sgs_valid_ready = true;
   }
   const double dx = c.xlen / c.nx, dy = c.ylen / c.ny, dz = c.zlen / c.nz;
-  InferArgs A{};
-  A.F = ctx->d_les;
+  InferArgs<T> A{};
+  A.F = (const T *)ctx->d_les;
   A.nxh = c.nx + 2; A.nyh = c.ny + 2; A.nh = A.nxh * A.nyh * (c.nz + 2); A.n3 = n3;
   A.nx = c.nx; A.ny = c.ny; A.nz = c.nz;
   A.valid = ctx->d_sgs_valid;
@@ -597,11 +654,15 @@ int sgs_infer_device(pu_ctx *ctx, double dt) {
   const double vmx = 0.25 * dx * dx / dt, vmy = 0.25 * dy * dy / dt, vmz = 0.25 * dz * dz / dt;
   A.nu_max = 0.5 * std::fmin(vmx, std::fmin(vmy, vmz));
   A.tau = ctx->d_sgs_tau; A.flags = ctx->d_sgs_flags;
-  const long long ngroups = (n3 + 63) / 64;
+  const long long ngroups = (long long)((c.nx + GX - 1) / GX) * c.ny * c.nz;
   const unsigned grid = (unsigned)std::min<long long>(ngroups, 256LL * 16);
-  hipLaunchKernelGGL(k_sgs_infer, dim3(grid), dim3(256), 0, ctx->stream, A);
+  hipLaunchKernelGGL(k_sgs_infer<T>, dim3(grid), dim3(256), 0, ctx->stream, A);
   PU_HIP_CHECK(hipGetLastError());
   return PU_OK;
+}
+
+int sgs_infer_device(pu_ctx *ctx, double dt) {
+  return ctx->cfg.precision == PU_FP32 ? sgs_infer_t<float>(ctx, dt) : sgs_infer_t<double>(ctx, dt);
 }
 
 }  // namespace pu

@@ -104,6 +104,24 @@ def test_les_matches_oracle():
     assert nrm.max() < 1e-12, nrm
 
 
+def test_les_fp32_matches_oracle():
+    """FP32 dycore mode: the closure's box, fluxes and tendencies are float, applied as increments
+    to the FP64 entries.  Each field's increment agrees with the FP64 oracle's to 1e-4 of its
+    largest value (float rounding of the box values and fluxes, ~1e-6 relative, with margin)."""
+    cfg = small_config(nx=22, ny=20, nz=16, ord=5, tracers=[WV, TKE, P1, P2], precision=pa.PU_FP32)
+    st, imm = random_state(cfg, seed=8, buildings=3)
+    d, cs = gpu_setup(cfg, st, imm)
+    d.les_apply(cs, 0.3)
+    ref = st.copy()
+    oracle_py.les_apply(cfg, ref, imm, 0.3)
+    g = cs.download()
+    assert np.array_equal(g[0], st[0])                    # density is not touched
+    for f in range(1, g.shape[0]):
+        inc_r, inc_g = ref[f] - st[f], g[f] - st[f]
+        scale = max(np.abs(inc_r).max(), 1e-300)
+        assert np.abs(inc_g - inc_r).max() <= 1e-4 * scale, f
+
+
 def test_mass_conserved_on_device():
     cfg = small_config(nx=24, ny=20, nz=18, ord=9, tracers=[WV, P1])
     st, imm = random_state(cfg, seed=4)
@@ -162,3 +180,23 @@ def test_case_init_bubble_closed_form():
     assert theta.min() == pytest.approx(300., abs=1e-9)
     assert 301.5 < theta.max() <= 302.0 + 1e-9
     assert np.abs(st[1:4]).max() == 0
+
+
+def test_surface_fluxes_match_oracle():
+    cfg = small_config(nx=22, ny=20, nz=16, ord=5, tracers=[WV, TKE, P1], dx=3.)
+    st, imm = random_state(cfg, seed=31, buildings=3)
+    rng = np.random.default_rng(32)
+    n = (cfg["nz"], cfg["ny"], cfg["nx"])
+    rough = rng.uniform(0.05, 0.5, n)
+    temp = np.where(rng.random(n) < 0.5, 0., rng.uniform(280., 320., n))
+    sfc_rough = rng.uniform(0.05, 0.5, n[1:])
+    sfc_temp = rng.uniform(285., 300., n[1:])
+    d, cs = gpu_setup(cfg, st, imm)
+    d.surface_init(rough, temp, sfc_rough, sfc_temp)
+    d.surface_apply(cs, 0.7)
+    ref = st.copy()
+    oracle_py.surface_fluxes(cfg, ref, imm, rough, temp, sfc_rough, sfc_temp, 0.7)
+    g = cs.download()
+    assert np.abs(g[1:5] - st[1:5]).max() > 0
+    nrm = rel_norms(g, ref)
+    assert nrm.max() < 1e-13, nrm
