@@ -279,22 +279,21 @@ template <class ST> struct FuseArgs {
   CombineParams C;
 };
 
+// un, us: the field's values in U and (stages 1, 2) Us, read by the caller ahead of time
 template <class ST>
-__device__ __forceinline__ void combine_field(const FuseArgs<ST> &F, int l, long long id, double tn, double rho_in,
-                                              double u_in, double v_in, double mult, double tr_r, double tr_rt) {
+__device__ __forceinline__ void combine_field_v(const FuseArgs<ST> &F, int l, long long id, double tn, double un,
+                                                double us, double rho_in, double u_in, double v_in, double mult,
+                                                double tr_r, double tr_rt) {
   const CombineParams &P = F.C;
   if (l == idW && P.enable_gravity) tn += -P.grav * (rho_in - tr_r);
   if (P.coriolis && l == idU) tn += P.fcor * v_in;
   if (P.coriolis && l == idV) tn -= P.fcor * u_in;
-  double un = load_cons(F.U, F.Ulo, l, id, P.n3);
   double v;
   if (P.stage == 0) {
     v = un + P.dt * tn;
   } else if (P.stage == 1) {
-    double us = load_cons(F.Us, F.Uslo, l, id, P.n3);
     v = (3. / 4.) * un + (1. / 4.) * us + (1. / 4.) * P.dt * tn;
   } else {
-    double us = load_cons(F.Us, F.Uslo, l, id, P.n3);
     v = (1. / 3.) * un + (2. / 3.) * us + (2. / 3.) * P.dt * tn;
   }
   bool positive = l >= NUM_STATE && ((P.positive_mask >> (l - NUM_STATE)) & 1u);
@@ -364,7 +363,7 @@ __device__ __forceinline__ Upw<T> riemann(T rL, T uL, T pL, T rR, T uR, T pR, T 
 }
 
 // MODE: 0 = first sweep of the stage (writes the tendencies), 1 = adds to them, 2 = last sweep:
-// each cell's final tendency goes straight into the stage combination (combine_field) instead of
+// each cell's final tendency goes straight into the stage combination (combine_field_v) instead of
 // back to `tend`.  The stage runs z (0), y (1), x (2): the last sweep is the one whose rows are
 // 64 cells wide, so the stage-state reads and writes of the combination are coalesced.
 template <int ORD, class T, int DIR, int MODE>
@@ -452,13 +451,31 @@ __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep
   if constexpr (MODE == 2) {
     if (has_tend) stage_inputs(FA, cid, k, f_rho, f_u, f_v, f_mult, f_trr, f_trt);
   }
-  auto accumulate = [&](int fidx, T v) {
+  // FP64: the values a cell's finished tendency is combined with are loaded before the field's
+  // reconstruction, so their latency hides behind it (measured 3.5 ms/step faster at cfg 2).  FP32:
+  // loaded when needed -- the extra registers cost the FP32 kernel one wave of occupancy (slower).
+  constexpr bool PREFETCH = std::is_same<T, double>::value;
+  struct Pre { T t; double u, us; };
+  auto load_pre = [&](int fidx) {
+    Pre p{T(0), 0., 0.};
+    if (has_tend) {
+      if constexpr (MODE >= 1) p.t = tend[(long long)fidx * P.n3 + cid];
+      if constexpr (MODE == 2) {
+        p.u = load_cons(FA.U, FA.Ulo, fidx, cid, P.n3);
+        if (FA.C.stage > 0) p.us = load_cons(FA.Us, FA.Uslo, fidx, cid, P.n3);
+      }
+    }
+    return p;
+  };
+  auto prefetch = [&](int fidx) { return PREFETCH ? load_pre(fidx) : Pre{T(0), 0., 0.}; };
+  auto accumulate = [&](int fidx, T v, const Pre &pf0) {
     T *tp = tend + (long long)fidx * P.n3 + cid;
+    const Pre pf = PREFETCH ? pf0 : load_pre(fidx);
     if constexpr (MODE == 2) {
-      const T tsum = *tp + v;
-      combine_field(FA, fidx, cid, (double)tsum, f_rho, f_u, f_v, f_mult, f_trr, f_trt);
+      const T tsum = pf.t + v;
+      combine_field_v(FA, fidx, cid, (double)tsum, pf.u, pf.us, f_rho, f_u, f_v, f_mult, f_trr, f_trt);
     } else if constexpr (MODE == 1) {
-      *tp = *tp + v;
+      *tp = pf.t + v;
     } else {
       *tp = v;
     }
@@ -474,6 +491,7 @@ __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep
   // ---- phase A: density, normal velocity, pressure -> acoustic upwinding ----
   constexpr int ln = 1 + DIR;
   T qLr = 0, qRr = 0, qLu = 0, qRu = 0, qLp = 0, qRp = 0;
+  const Pre preR = prefetch(idR), preU = prefetch(ln);
   if (do_recon) {
     recon(idR, false, qLr, qRr);
     recon(ln, false, qLu, qRu);
@@ -496,8 +514,8 @@ __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep
   bufB0[lin] = Fr; bufB1[lin] = Fu;
   __syncthreads();
   if (has_tend) {
-    accumulate(idR, -(bufB0[lin + nbs] - bufB0[lin]) * rd);
-    accumulate(ln, -(bufB1[lin + nbs] - bufB1[lin]) * rd);
+    accumulate(idR, -(bufB0[lin + nbs] - bufB0[lin]) * rd, preR);
+    accumulate(ln, -(bufB1[lin + nbs] - bufB1[lin]) * rd, preU);
   }
 
   // ---- phase B: the advected fields, one at a time ----
@@ -512,6 +530,7 @@ __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep
     else if (b == 2) { l = idT; fidx = idT; }
     else { l = NUM_STATE + 1 + (b - 3); fidx = NUM_STATE + (b - 3); }
     T qL = 0, qR = 0;
+    const Pre pf = prefetch(fidx);
     if (do_recon) recon(l, modify, qL, qR);
     bufA0[lin] = qR;
     __syncthreads();
@@ -526,7 +545,7 @@ __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep
     }
     bufB0[lin] = F;
     __syncthreads();
-    if (has_tend) accumulate(fidx, -(bufB0[lin + nbs] - bufB0[lin]) * rd);
+    if (has_tend) accumulate(fidx, -(bufB0[lin + nbs] - bufB0[lin]) * rd, pf);
   }
 }
 

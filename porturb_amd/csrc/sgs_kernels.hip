@@ -478,22 +478,26 @@ __global__ __launch_bounds__(256) void k_sgs_infer(InferArgs<T> A) {
   const long long ngroups = ngx * A.ny * A.nz;
   for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int i0 = (int)(grp % ngx) * GX, j = (int)((grp / ngx) % A.ny), k = (int)(grp / (ngx * A.ny));
-    // all loads of the staging issued before any LDS write (one wait, not one per element)
-    constexpr int PER = (4 * GBN + 255) / 256;
-    T buf[PER];
+    const int lxmax = (int)(A.nxh - i0);        // box x positions inside the array
+    // Staging: the box is 36 rows (quantity q, z offset, y offset) of 66 points; wave w loads rows
+    // w, w+4, ... with one lane per x point (lanes 0, 1 also take points 64, 65), so each row's
+    // address is wave-uniform.  All loads are issued before any LDS write.
+    constexpr int RPW = 36 / 4;
+    T buf[RPW], buf2[RPW];
 #pragma unroll
-    for (int e = 0; e < PER; e++) {
-      const int p = tid + 256 * e;
-      const int q = p / GBN, r = p % GBN;
-      const int lx = r % GBX, ly = (r / GBX) % 3, lz = r / (3 * GBX);
-      const long long ih = i0 + lx, jh = j + ly, kh = k + lz;       // closure-box coordinates
-      buf[e] = (p < 4 * GBN && ih < A.nxh) ? A.F[(long long)(1 + q) * A.nh + (kh * A.nyh + jh) * A.nxh + ih] : T(0);
+    for (int rr = 0; rr < RPW; rr++) {
+      const int r = w + 4 * rr;
+      const int q = r / 9, lz = (r / 3) % 3, ly = r % 3;
+      const T *row = A.F + (long long)(1 + q) * A.nh + ((long long)(k + lz) * A.nyh + (j + ly)) * A.nxh + i0;
+      buf[rr] = lane < lxmax ? row[lane] : T(0);
+      buf2[rr] = (lane < 2 && 64 + lane < lxmax) ? row[64 + lane] : T(0);
     }
     __syncthreads();                            // the previous group is done with sQ and sY
 #pragma unroll
-    for (int e = 0; e < PER; e++) {
-      const int p = tid + 256 * e;
-      if (p < 4 * GBN) sQ[p] = buf[e];
+    for (int rr = 0; rr < RPW; rr++) {
+      const int r = w + 4 * rr;
+      sQ[r * GBX + lane] = buf[rr];
+      if (lane < 2) sQ[r * GBX + 64 + lane] = buf2[rr];
     }
     __syncthreads();
     const int xl = 16 * w + r16;                // this lane's cell within the group
