@@ -46,6 +46,26 @@ __device__ __forceinline__ fx4 mfma16(bfx4 a, bfx4 b, fx4 c) {
 
 __device__ __forceinline__ bfx4 ld4(const short *p) { return *reinterpret_cast<const bfx4 *>(p); }
 
+// gfx950 v_mfma_f32_16x16x32_bf16: lane l holds A[row l%16][k = 8*(l/16) .. +7],
+// B[k = 8*(l/16) .. +7][col l%16]; C/D as for 16x16x16.
+typedef short bfx8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ fx4 mfma32(bfx8 a, bfx8 b, fx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                  0, 0, 0);
+}
+__device__ __forceinline__ bfx8 ld8(const short *p) { return *reinterpret_cast<const bfx8 *>(p); }
+__device__ __forceinline__ bfx8 cat8(bfx4 lo, bfx4 hi) {
+  return bfx8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
+// Input order of the 64-unit layers in the 16x16x32 inference: position k of a 32-wide K block
+// kb holds hidden unit 16 (2 kb + (k%8 >= 4)) + 4 (k%32 / 8) + k%4, so the two 16-unit output
+// fragments of the previous layer, side by side, are exactly the lane's 8 B-operand values.
+__host__ __device__ constexpr int unit_of_k(int k) {
+  return 16 * (2 * (k / 32) + ((k % 8) >= 4 ? 1 : 0)) + 4 * ((k % 32) / 8) + (k % 4);
+}
+
 // GELU(x) = x Phi(x), Phi(x) = (1 + erf(x / sqrt 2)) / 2, with erf from Abramowitz & Stegun,
 // Handbook of Mathematical Functions, formula 7.1.26 (|error| <= 1.5e-7, far below the BF16
 // rounding of the activations): for z >= 0, erf(z) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5))))
@@ -456,24 +476,23 @@ template <class T> struct InferArgs {
 constexpr int GX = 64, GBX = GX + 2, GBN = GBX * 3 * 3;
 
 template <class T>
-__global__ __launch_bounds__(256) void k_sgs_infer(InferArgs<T> A) {
-  __shared__ __align__(16) short sW1[NH * NINP];
-  __shared__ __align__(16) short sW2[NH * NH];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sgs_infer(InferArgs<T> A) {
+  constexpr int K1 = 128;                       // layer-1 inputs: 4 K blocks of 32 (108 used)
+  __shared__ __align__(16) short sW1[NH * NINP];   // row stride 112: block 3 reads 16 columns
+  __shared__ __align__(16) short sW2[NH * NH];  // columns in unit_of_k order
   __shared__ __align__(16) short sW3[NOUTP * NH];
   __shared__ float sB[NH + NH + NOUTP];
-  __shared__ float sFm[NINP];
+  __shared__ float sFm[K1];
   __shared__ float sY[4][16][NOUTP + 1];
   __shared__ T sQ[4 * GBN];                     // u, v, w, theta of the group's neighbourhood
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, r16 = lane & 15, g = lane >> 4;
   for (int p = tid; p < NH * NINP / 4; p += 256)
     reinterpret_cast<uint2 *>(sW1)[p] = reinterpret_cast<const uint2 *>(A.wbf + P_W1)[p];
-  for (int p = tid; p < NH * NH / 4; p += 256)
-    reinterpret_cast<uint2 *>(sW2)[p] = reinterpret_cast<const uint2 *>(A.wbf + P_W2)[p];
-  for (int p = tid; p < NOUTP * NH / 4; p += 256)
-    reinterpret_cast<uint2 *>(sW3)[p] = reinterpret_cast<const uint2 *>(A.wbf + P_W3)[p];
+  for (int p = tid; p < NH * NH; p += 256) sW2[p] = A.wbf[P_W2 + (p / NH) * NH + unit_of_k(p % NH)];
+  for (int p = tid; p < NOUTP * NH; p += 256) sW3[p] = A.wbf[P_W3 + (p / NH) * NH + unit_of_k(p % NH)];
   for (int p = tid; p < NH + NH + NOUTP; p += 256)
     sB[p] = p < NH ? A.master[P_B1 + p] : (p < 2 * NH ? A.master[P_B2 + p - NH] : A.master[P_B3 + p - 2 * NH]);
-  for (int p = tid; p < NINP; p += 256) sFm[p] = p < NIN ? A.fmul[p] : 0.f;
+  for (int p = tid; p < K1; p += 256) sFm[p] = p < NIN ? A.fmul[p] : 0.f;
   const long long ngx = (A.nx + GX - 1) / GX;
   const long long ngroups = ngx * A.ny * A.nz;
   for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
@@ -504,38 +523,57 @@ __global__ __launch_bounds__(256) void k_sgs_infer(InferArgs<T> A) {
     const int i = i0 + xl;
     const bool in = i < A.nx;
     const int c = (1 * 3 + 1) * GBX + xl + 1;  // the cell's index in the staged box
-    bfx4 xb[NINP / 16];
+    // inputs: K block kb, lane group g -> neighbours 8 kb + 2 g and 8 kb + 2 g + 1
+    bfx8 xb[4];
     float mx = 0.f;
-    for (int kb = 0; kb < NINP / 16; kb++) {
-      const int n = 4 * kb + g;
-      bfx4 v = {0, 0, 0, 0};
-      if (n < 27) {
-        const int dz = n / 9 - 1, dy = (n / 3) % 3 - 1, dx = n % 3 - 1;
-        const int nb = c + (dz * 3 + dy) * GBX + dx;
-        for (int q = 0; q < 4; q++) {
-          const float x = (float)(sQ[q * GBN + nb] - sQ[q * GBN + c]) * sFm[4 * n + q];
-          mx = fmaxf(mx, fabsf(x));
-          v[q] = bf16_bits(x);
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) {
+      bfx8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int n = 8 * kb + 2 * g + h;
+        if (n < 27) {
+          const int dz = n / 9 - 1, dy = (n / 3) % 3 - 1, dx = n % 3 - 1;
+          const int nb = c + (dz * 3 + dy) * GBX + dx;
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const float x = (float)(sQ[q * GBN + nb] - sQ[q * GBN + c]) * sFm[4 * n + q];
+            mx = fmaxf(mx, fabsf(x));
+            v[4 * h + q] = bf16_bits(x);
+          }
         }
       }
       xb[kb] = v;
     }
     mx = fmaxf(mx, __shfl_xor(mx, 16));
     mx = fmaxf(mx, __shfl_xor(mx, 32));
-    // layer 1 -> a1 (B fragments of layer 2)
+    // layer 1: D tiles of units 16 mt .. 16 mt + 15
     bfx4 a1[4], a2[4];
+#pragma unroll
     for (int mt = 0; mt < 4; mt++) {
       fx4 acc = {0.f, 0.f, 0.f, 0.f};
-      for (int kb = 0; kb < NINP / 16; kb++) acc = mfma16(ld4(sW1 + (16 * mt + r16) * NINP + 16 * kb + 4 * g), xb[kb], acc);
+#pragma unroll
+      for (int kb = 0; kb < 4; kb++) {
+        // inputs 112..127 do not exist (their B values are 0): lanes 32..63 of block 3 use a zero A
+        const bfx8 wa = (kb == 3 && g >= 2) ? bfx8{0, 0, 0, 0, 0, 0, 0, 0} : ld8(sW1 + (16 * mt + r16) * NINP + 32 * kb + 8 * g);
+        acc = mfma32(wa, xb[kb], acc);
+      }
+#pragma unroll
       for (int r = 0; r < 4; r++) a1[mt][r] = bf16_bits(gelu(acc[r] + sB[16 * mt + 4 * g + r]));
     }
+    // layer 2: the B fragment of K block kb2 is (a1[2 kb2], a1[2 kb2 + 1]) in unit_of_k order
+#pragma unroll
     for (int mt = 0; mt < 4; mt++) {
       fx4 acc = {0.f, 0.f, 0.f, 0.f};
-      for (int kb = 0; kb < 4; kb++) acc = mfma16(ld4(sW2 + (16 * mt + r16) * NH + 16 * kb + 4 * g), a1[kb], acc);
+#pragma unroll
+      for (int kb = 0; kb < 2; kb++)
+        acc = mfma32(ld8(sW2 + (16 * mt + r16) * NH + 32 * kb + 8 * g), cat8(a1[2 * kb], a1[2 * kb + 1]), acc);
+#pragma unroll
       for (int r = 0; r < 4; r++) a2[mt][r] = bf16_bits(gelu(acc[r] + sB[NH + 16 * mt + 4 * g + r]));
     }
     fx4 y = {0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < 4; kb++) y = mfma16(ld4(sW3 + r16 * NH + 16 * kb + 4 * g), a2[kb], y);
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++) y = mfma32(ld8(sW3 + r16 * NH + 32 * kb + 8 * g), cat8(a2[2 * kb], a2[2 * kb + 1]), y);
     for (int r = 0; r < 4; r++) sY[w][r16][4 * g + r] = y[r] + sB[2 * NH + 4 * g + r];
     __syncthreads();
     if (g == 0 && in) {
