@@ -123,6 +123,28 @@ int  pu_surface_fluxes_init(pu_ctx *ctx, const double *immersed_roughness, const
                             const double *surface_roughness, const double *surface_temp);
 int  pu_surface_fluxes_apply(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys);
 
+/* modules::sponge_layer (model/modules/sponge_layer.h:7-60): above z1 = (1 - top_prop) zlen, u, v
+ * and temp relax towards their horizontal means at each level and w towards 0 with
+ * factor = ((z - z1)/(zlen - z1))^3 dt / time_scale.  The means cover the levels from
+ * k1 = floor(z1/dz - 0.5) up (k1 below 0 starts at level 0: no cell below the domain has z > z1).
+ * Requires time_scale > 0 and 0 < top_prop <= 1.  Stream-ordered. */
+int  pu_sponge_layer(pu_ctx *ctx, pu_coupler_state *cs, double dt, double time_scale, double top_prop);
+
+/* modules::geostrophic_wind_forcing (model/modules/geostrophic_wind_forcing.h:7-36):
+ * f = 2 * 7.2921e-5 * sin(lat_g pi/180); with <u>(k), <v>(k) the horizontal means per level,
+ * uvel += dt f (<v> - v_g), vvel -= dt f (<u> - u_g).  Stream-ordered. */
+int  pu_geostrophic_wind_forcing(pu_ctx *ctx, pu_coupler_state *cs, double dt, double lat_g, double u_g, double v_g);
+
+/* Horizontal means of uvel and vvel on levels k_begin .. k_end-1, to host arrays of k_end - k_begin
+ * values: the sums of modules::uniform_pg_wind_forcing (uniform_pg_wind_forcing.h:18-21, domain
+ * mean = mean of the level means) and uniform_pg_wind_forcing_height (:56-60).  Synchronous. */
+int  pu_level_mean_wind(pu_ctx *ctx, const pu_coupler_state *cs, int k_begin, int k_end, double *u_mean,
+                        double *v_mean);
+
+/* uvel += du, vvel += dv on every cell: the update of uniform_pg_wind_forcing(_height)
+ * (uniform_pg_wind_forcing.h:31-34,75-78).  Stream-ordered. */
+int  pu_add_uniform_wind(pu_ctx *ctx, pu_coupler_state *cs, double du, double dv);
+
 /* FP64 reductions on the device: total mass sum(rho)*dV (rho = dry + mass-adding tracers), max
  * wind speed and max |w| (the quantities of Coupler::inform_user, coupler.h:507-530). */
 int  pu_diagnostics(pu_ctx *ctx, const pu_coupler_state *cs, double *mass, double *max_wind,
@@ -160,12 +182,30 @@ int  pu_kernel_time(pu_ctx *ctx, const char *name, double *total_ms, long long *
 /* Benchmark driver on the drop-in C++ API (core::Coupler + modules), the analog of the reference's
  * experiments/examples/abl_neutral.cpp:65-179 main loop: constants as sc_init.h:31-51, tracers
  * water_vapor (+ TKE from LES_Closure::init when closure != 0, + passive_1/2 for the ABL cases),
- * pu_case_init, dycore init; pu_driver_step runs run_module(dycore.time_step) and
- * run_module(les_closure.apply) with dt = compute_time_step, nsteps times.  sgs_model (NULL or a
- * network file, with closure != 0) sets option les_sgs_model: the learned subgrid closure. */
+ * pu_case_init, dycore init; pu_driver_step runs, with dt = compute_time_step, nsteps times:
+ * run_module(geostrophic_wind_forcing) and run_module(uniform_pg_wind_forcing[_height]) when
+ * enabled, run_module(dycore.time_step), run_module(sponge_layer) when enabled,
+ * run_module(apply_surface_fluxes) for the ABL cases and run_module(les_closure.apply): the module
+ * order of abl_neutral.cpp:98-103 and city.cpp:130-141.  sgs_model (NULL or a network file, with
+ * closure != 0) sets option les_sgs_model: the learned subgrid closure. */
 typedef struct pu_driver pu_driver;
 int  pu_driver_create(int case_id, int nx, int ny, int nz, double dx, int ord, int precision,
                       unsigned long long seed, int closure, const char *sgs_model, pu_driver **out);
+/* Column forcing of the driver step (off by default).  geostrophic: geostrophic_wind_forcing(c, dt,
+ * lat_g, u_g, v_g) as abl_neutral.cpp:93; sponge: sponge_layer(c, dt, sponge_dt_multiple * dt,
+ * sponge_top_prop) as abl_neutral.cpp:95 (100, 0.1); pg = 1: uniform_pg_wind_forcing(c, dt, pg_tau,
+ * pg_update_cycles), pg = 2: uniform_pg_wind_forcing_height(c, dt, pg_height, pg_u0, pg_v0, pg_tau)
+ * as city.cpp:131-136. */
+typedef struct {
+  int    geostrophic;
+  double lat_g, u_g, v_g;
+  int    sponge;
+  double sponge_dt_multiple, sponge_top_prop;
+  int    pg;
+  double pg_height, pg_u0, pg_v0, pg_tau;
+  unsigned long pg_update_cycles;
+} pu_driver_forcing;
+int  pu_driver_set_forcing(pu_driver *d, const pu_driver_forcing *f);
 int  pu_driver_step(pu_driver *d, int nsteps);
 int  pu_driver_state(pu_driver *d, pu_coupler_state *out, int *num_tracers);
 pu_ctx *pu_driver_context(pu_driver *d);

@@ -26,7 +26,7 @@ struct PortUrbError : std::runtime_error {
 
 class Options {
  public:
-  using Value = std::variant<bool, int, long long, float, double, std::string>;
+  using Value = std::variant<bool, int, long long, unsigned long, float, double, std::string>;
 
   template <class T> void add_option(std::string key, T value) {
     if (option_exists(key)) endrun("ERROR: Options::add_option: key [" + key + "] already exists");

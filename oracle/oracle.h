@@ -132,6 +132,18 @@ int    oc_les_apply_learned(const oc_config *cfg, double *coupler, const double 
 int    oc_surface_fluxes(const oc_config *c, double *coupler, const double *prop, const double *rough,
                          const double *temp, const double *sfc_rough, const double *sfc_temp, double dt);
 
+/* Column-mean forcing modules (column.c), in place on the coupler state. */
+typedef struct {
+  unsigned long counter;     /* option uniform_pg_wind_forcing_counter */
+  double u0, v0, u, v;       /* options uniform_pg_wind_forcing_{u0,v0,u,v} */
+} oc_pg_state;
+int    oc_sponge_layer(const oc_config *c, double *coupler, double dt, double time_scale, double top_prop);
+int    oc_geostrophic_forcing(const oc_config *c, double *coupler, double dt, double lat_g, double u_g, double v_g);
+int    oc_uniform_pg_forcing(const oc_config *c, double *coupler, double dt, double tau, unsigned long update_cycles,
+                             oc_pg_state *st);
+int    oc_uniform_pg_forcing_height(const oc_config *c, double *coupler, double dt, double height, double u0,
+                                    double v0, double tau);
+
 /* Sum over cells of the dycore's full density rho (dry + mass-adding tracers), times cell volume. */
 double oc_total_mass(const oc_config *cfg, const double *coupler);
 

@@ -99,6 +99,14 @@ def lib():
         L.oc_les_apply_learned.restype = I
         L.oc_surface_fluxes.argtypes = [cp, P, P, P, P, P, P, D]
         L.oc_surface_fluxes.restype = I
+        L.oc_sponge_layer.argtypes = [cp, P, D, D, D]
+        L.oc_sponge_layer.restype = I
+        L.oc_geostrophic_forcing.argtypes = [cp, P, D, D, D, D]
+        L.oc_geostrophic_forcing.restype = I
+        L.oc_uniform_pg_forcing.argtypes = [cp, P, D, D, C.c_ulong, C.POINTER(OcPgState)]
+        L.oc_uniform_pg_forcing.restype = I
+        L.oc_uniform_pg_forcing_height.argtypes = [cp, P, D, D, D, D, D]
+        L.oc_uniform_pg_forcing_height.restype = I
         _lib = L
     return _lib
 
@@ -299,6 +307,39 @@ def les_apply_learned(cfg, coupler, imm, dt, model, bf16=True, nthreads=0):
     assert lib().oc_les_apply_learned(C.byref(oc), coupler.ctypes.data, imm.ctypes.data, dt, C.byref(model.s),
                                       int(bf16), tau.ctypes.data, flags.ctypes.data) == 0
     return tau, flags
+
+
+class OcPgState(C.Structure):
+    """Options the uniform pressure-gradient forcing keeps between calls."""
+    _fields_ = [("counter", C.c_ulong), ("u0", C.c_double), ("v0", C.c_double), ("u", C.c_double),
+                ("v", C.c_double)]
+
+
+def _coupler_call(fn, cfg, coupler, *args):
+    oc = to_oc_config(cfg)
+    assert coupler.flags["C_CONTIGUOUS"] and coupler.dtype == np.float64
+    return fn(C.byref(oc), coupler.ctypes.data, *args)
+
+
+def sponge_layer(cfg, coupler, dt, time_scale, top_prop):
+    """modules::sponge_layer in place on the coupler state."""
+    assert _coupler_call(lib().oc_sponge_layer, cfg, coupler, dt, time_scale, top_prop) == 0
+
+
+def geostrophic_forcing(cfg, coupler, dt, lat_g, u_g, v_g):
+    """modules::geostrophic_wind_forcing in place on the coupler state."""
+    assert _coupler_call(lib().oc_geostrophic_forcing, cfg, coupler, dt, lat_g, u_g, v_g) == 0
+
+
+def uniform_pg_forcing(cfg, coupler, dt, tau, update_cycles, state):
+    """modules::uniform_pg_wind_forcing in place; `state` (OcPgState) carries the options."""
+    assert _coupler_call(lib().oc_uniform_pg_forcing, cfg, coupler, dt, tau, update_cycles, C.byref(state)) == 0
+
+
+def uniform_pg_forcing_height(cfg, coupler, dt, height, u0, v0, tau):
+    """modules::uniform_pg_wind_forcing_height in place; returns False for a height the
+    reference cannot serve (below the first cell centre)."""
+    return _coupler_call(lib().oc_uniform_pg_forcing_height, cfg, coupler, dt, height, u0, v0, tau) == 0
 
 
 def surface_fluxes(cfg, coupler, prop, rough, temp, sfc_rough, sfc_temp, dt):

@@ -92,6 +92,7 @@ def lib():
         "pu_memcpy_d2d": (I, [P, P, S]),
         "pu_driver_create": (I, [I, I, I, I, D, I, I, C.c_ulonglong, I, C.c_char_p, C.POINTER(P)]),
         "pu_driver_step": (I, [P, I]),
+        "pu_driver_set_forcing": (I, [P, P]),
         "pu_driver_state": (I, [P, csp, C.POINTER(I)]),
         "pu_driver_context": (P, [P]),
         "pu_driver_tracer_name": (I, [P, I, C.c_char_p, I]),
@@ -116,6 +117,10 @@ def lib():
         "pu_les_closure_sgs": (I, [P, csp, D, P, P]),
         "pu_surface_fluxes_init": (I, [P, P, P, P, P]),
         "pu_surface_fluxes_apply": (I, [P, csp, D]),
+        "pu_sponge_layer": (I, [P, csp, D, D, D]),
+        "pu_geostrophic_wind_forcing": (I, [P, csp, D, D, D, D]),
+        "pu_level_mean_wind": (I, [P, csp, I, I, P, P]),
+        "pu_add_uniform_wind": (I, [P, csp, D, D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -315,6 +320,22 @@ class Dycore:
     def surface_apply(self, cs, dt):
         check(lib().pu_surface_fluxes_apply(self.ctx, C.byref(cs.c), dt))
 
+    def sponge_layer(self, cs, dt, time_scale, top_prop):
+        check(lib().pu_sponge_layer(self.ctx, C.byref(cs.c), dt, time_scale, top_prop))
+
+    def geostrophic_forcing(self, cs, dt, lat_g, u_g, v_g):
+        check(lib().pu_geostrophic_wind_forcing(self.ctx, C.byref(cs.c), dt, lat_g, u_g, v_g))
+
+    def level_mean_wind(self, cs, k_begin, k_end):
+        """Horizontal means of uvel and vvel on levels k_begin .. k_end-1 (host arrays)."""
+        n = max(k_end - k_begin, 0)
+        u, v = np.zeros(n), np.zeros(n)
+        check(lib().pu_level_mean_wind(self.ctx, C.byref(cs.c), k_begin, k_end, u.ctypes.data, v.ctypes.data))
+        return u, v
+
+    def add_uniform_wind(self, cs, du, dv):
+        check(lib().pu_add_uniform_wind(self.ctx, C.byref(cs.c), du, dv))
+
     def set_sgs_model(self, mlp):
         check(lib().pu_les_closure_set_model(self.ctx, mlp.h if mlp is not None else None))
 
@@ -498,6 +519,13 @@ def bf16_to_float(a):
     return (np.asarray(a).astype(np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
+class PuDriverForcing(C.Structure):
+    _fields_ = [("geostrophic", C.c_int), ("lat_g", C.c_double), ("u_g", C.c_double), ("v_g", C.c_double),
+                ("sponge", C.c_int), ("sponge_dt_multiple", C.c_double), ("sponge_top_prop", C.c_double),
+                ("pg", C.c_int), ("pg_height", C.c_double), ("pg_u0", C.c_double), ("pg_v0", C.c_double),
+                ("pg_tau", C.c_double), ("pg_update_cycles", C.c_ulong)]
+
+
 class Driver:
     """The C++ benchmark driver (core::Coupler + Dynamics_Euler_Stratified_WenoFV + LES_Closure)
     for one case; the coupler's DataManager owns the state."""
@@ -512,6 +540,13 @@ class Driver:
         self.d = p.value
         self.shape = (nz, ny, nx)
         self.ctx = lib().pu_driver_context(self.d)
+
+    def set_forcing(self, **kw):
+        """Column forcing of the step: geostrophic (lat_g, u_g, v_g), sponge (sponge_dt_multiple,
+        sponge_top_prop), pg = 1 (pg_tau, pg_update_cycles) or 2 (pg_height, pg_u0, pg_v0, pg_tau)."""
+        f = PuDriverForcing(**kw)
+        if lib().pu_driver_set_forcing(self.d, C.byref(f)) != PU_OK:
+            raise ValueError("pu_driver_set_forcing: " + lib().pu_driver_last_error().decode())
 
     def step(self, n=1):
         if lib().pu_driver_step(self.d, n) != PU_OK:

@@ -62,7 +62,8 @@ template <class P> static int alloc(P **p, size_t bytes) {
 static void free_ctx(pu_ctx *c) {
   void *ptrs[] = {c->d_hy, c->d_imm_flag, c->d_any10, c->d_tau, c->d_relax, c->d_imm_les, c->d_imm_les_val,
                   c->d_U, c->d_Us, c->d_Ulo, c->d_Uslo, c->d_tend, c->d_prim, c->d_les, c->d_red,
-                  c->d_sgs_tau, c->d_sgs_flags, c->d_sgs_valid, c->d_sgs_fmul, c->d_sfc_rough, c->d_sfc_temp};
+                  c->d_sgs_tau, c->d_sgs_flags, c->d_sgs_valid, c->d_sgs_fmul, c->d_sfc_rough, c->d_sfc_temp,
+                  c->d_col};
   for (void *p : ptrs) if (p) hipFree(p);
   if (c->t0) hipEventDestroy(c->t0);
   if (c->t1) hipEventDestroy(c->t1);

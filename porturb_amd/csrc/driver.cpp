@@ -3,9 +3,12 @@
 // Mirrors the structure of the reference's ABL drivers (experiments/examples/abl_neutral.cpp:
 // 65-179): set options and constants as custom_modules::sc_init does (sc_init.h:31-51), allocate
 // the coupled state, add tracers, initialise the case, init the LES closure and the dycore, then
-// per step run_module(dycore.time_step), run_module(apply_surface_fluxes) for the rough-wall ABL
-// cases and run_module(les_closure.apply) with dt = dycore.compute_time_step (the reference's
-// module order, abl_neutral.cpp:98-103, without its geostrophic forcing, sponge and averager).  Exposed through pu_driver_* for bench.py and the tests.
+// per step, with dt = dycore.compute_time_step: the optional column forcing
+// (geostrophic_wind_forcing, uniform_pg_wind_forcing[_height]), run_module(dycore.time_step), the
+// optional sponge_layer, run_module(apply_surface_fluxes) for the rough-wall ABL cases and
+// run_module(les_closure.apply) -- the reference's module order (abl_neutral.cpp:98-103,
+// city.cpp:130-141) without its time averager.  Exposed through pu_driver_* for bench.py and the
+// tests.
 #include <cmath>
 #include <memory>
 #include <string>
@@ -13,8 +16,11 @@
 #include "porturb.h"
 #include "porturb/core/coupler.h"
 #include "porturb/modules/dynamics_rk.h"
+#include "porturb/modules/geostrophic_wind_forcing.h"
 #include "porturb/modules/les_closure.h"
+#include "porturb/modules/sponge_layer.h"
 #include "porturb/modules/surface_flux.h"
+#include "porturb/modules/uniform_pg_wind_forcing.h"
 
 struct pu_driver {
   core::Coupler coupler;
@@ -22,6 +28,7 @@ struct pu_driver {
   modules::LES_Closure les;
   bool closure = true;
   bool surface = false;     // rough-wall surface fluxes (the ABL cases)
+  pu_driver_forcing forcing{};
   std::string error;
 };
 
@@ -94,13 +101,38 @@ int pu_driver_create(int case_id, int nx, int ny, int nz, double dx, int ord, in
   }
 }
 
+int pu_driver_set_forcing(pu_driver *d, const pu_driver_forcing *f) {
+  if (!d || !f) return PU_ERR_ARG;
+  if (f->pg < 0 || f->pg > 2 || (f->pg == 1 && f->pg_update_cycles == 0) || (f->pg && !(f->pg_tau > 0)) ||
+      (f->sponge && (!(f->sponge_dt_multiple > 0) || !(f->sponge_top_prop > 0) || !(f->sponge_top_prop <= 1)))) {
+    g_driver_error = "pu_driver_set_forcing: invalid forcing parameters";
+    return PU_ERR_ARG;
+  }
+  d->forcing = *f;
+  return PU_OK;
+}
+
 int pu_driver_step(pu_driver *d, int nsteps) {
   if (!d) return PU_ERR_ARG;
   try {
     core::Coupler &c = d->coupler;
+    const pu_driver_forcing &f = d->forcing;
     for (int n = 0; n < nsteps; n++) {
       double dt = d->dycore.compute_time_step(c);
+      if (f.geostrophic)
+        c.run_module([&](core::Coupler &cc) { modules::geostrophic_wind_forcing(cc, dt, f.lat_g, f.u_g, f.v_g); },
+                     "geostrophic_forcing");
+      if (f.pg == 1)
+        c.run_module([&](core::Coupler &cc) { modules::uniform_pg_wind_forcing(cc, dt, f.pg_tau, f.pg_update_cycles); },
+                     "pg_forcing");
+      if (f.pg == 2)
+        c.run_module([&](core::Coupler &cc) {
+          modules::uniform_pg_wind_forcing_height(cc, dt, f.pg_height, f.pg_u0, f.pg_v0, f.pg_tau);
+        }, "pg_forcing");
       c.run_module([&](core::Coupler &cc) { d->dycore.time_step(cc, dt); }, "dycore");
+      if (f.sponge)
+        c.run_module([&](core::Coupler &cc) { modules::sponge_layer(cc, dt, f.sponge_dt_multiple * dt, f.sponge_top_prop); },
+                     "sponge");
       if (d->surface) c.run_module([&](core::Coupler &cc) { modules::apply_surface_fluxes(cc, dt); }, "surface_fluxes");
       if (d->closure) c.run_module([&](core::Coupler &cc) { d->les.apply(cc, dt); }, "les_closure");
       c.set_option<double>("elapsed_time", c.get_option<double>("elapsed_time") + dt);

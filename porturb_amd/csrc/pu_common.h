@@ -73,6 +73,8 @@ struct pu_ctx {
   // surface fluxes: halo'd immersed roughness and temperature boxes
   double  *d_sfc_rough = nullptr, *d_sfc_temp = nullptr;
   bool     sfc_ready = false;
+  // horizontal means per level (forcing modules): [3][nz] means + [3][nz][16] partial sums
+  double  *d_col = nullptr;
   // reductions
   double *d_red = nullptr;                     // scratch for reductions
   // timing

@@ -6,6 +6,10 @@
 #include "porturb/core/coupler.h"
 #include "porturb/modules/dynamics_rk.h"
 #include "porturb/modules/les_closure.h"
+#include "porturb/modules/geostrophic_wind_forcing.h"
+#include "porturb/modules/sponge_layer.h"
+#include "porturb/modules/surface_flux.h"
+#include "porturb/modules/uniform_pg_wind_forcing.h"
 
 static int fails = 0;
 #define EXPECT(c) do { if (!(c)) { std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, #c); fails++; } } while (0)
@@ -26,6 +30,8 @@ int main() {
   threw = false;
   try { o.add_option<double>("cfl", 0.5); } catch (core::PortUrbError const &) { threw = true; }
   EXPECT(threw);                                          // add of an existing key
+  o.set_option<size_t>("uniform_pg_wind_forcing_counter", 3);   // the counter type the forcing uses
+  EXPECT(o.get_option<size_t>("uniform_pg_wind_forcing_counter") == 3);
   core::Coupler c;
   c.set_option<double>("cfl", 0.3);
   EXPECT(c.get_option<double>("cfl", 0.6) == 0.3);
