@@ -145,6 +145,48 @@ int  pu_level_mean_wind(pu_ctx *ctx, const pu_coupler_state *cs, int k_begin, in
  * (uniform_pg_wind_forcing.h:31-34,75-78).  Stream-ordered. */
 int  pu_add_uniform_wind(pu_ctx *ctx, pu_coupler_state *cs, double du, double dv);
 
+/* ------------------------------------------------------------------------------------------
+ * Output / restart files: NetCDF classic with fixed-size variables, the format the reference's
+ * Coupler::write_output_file / overwrite_with_restart (model/core/coupler.h:533-723) write and
+ * read through yakl::SimplePNetCDF (create_dim, create_var, enddef, redef, write, write_all,
+ * read, read_all).  PU_NC_AUTO writes CDF-2 (64-bit offset) unless a variable needs CDF-5 (over
+ * 4 GiB, or an unsigned-byte / 64-bit integer type); the reference always writes CDF-5
+ * (NC_64BIT_DATA).  Reading accepts CDF-1, CDF-2 and CDF-5.  Device transfers (_device) take
+ * double arrays and convert/byte-swap on the GPU, on ctx's stream (ctx NULL: the default
+ * stream); they return after the data are in the file (put) or in device memory (get).  Host
+ * transfers take double, float, int or unsigned char (PU_NC_UBYTE) arrays. */
+enum { PU_NC_BYTE = 1, PU_NC_CHAR = 2, PU_NC_SHORT = 3, PU_NC_INT = 4, PU_NC_FLOAT = 5, PU_NC_DOUBLE = 6,
+       PU_NC_UBYTE = 7, PU_NC_INT64 = 10 };
+enum { PU_NC_AUTO = 0, PU_NC_CDF2 = 2, PU_NC_CDF5 = 5 };
+typedef struct pu_ncfile pu_ncfile;
+int  pu_nc_create(const char *path, int format, pu_ncfile **out);
+int  pu_nc_open(const char *path, pu_ncfile **out);
+int  pu_nc_def_dim(pu_ncfile *f, const char *name, long long len);
+int  pu_nc_def_var(pu_ncfile *f, const char *name, int nc_type, int ndims, const char *const *dimnames);
+int  pu_nc_redef(pu_ncfile *f);
+int  pu_nc_enddef(pu_ncfile *f);
+int  pu_nc_inq_var(pu_ncfile *f, const char *name, int *nc_type, int *ndims, long long *dims);
+int  pu_nc_put_host(pu_ncfile *f, const char *name, const void *data, int host_type);
+int  pu_nc_get_host(pu_ncfile *f, const char *name, void *data, int host_type);
+int  pu_nc_put_device(pu_ncfile *f, const char *name, const double *d_data, pu_ctx *ctx);
+int  pu_nc_get_device(pu_ncfile *f, const char *name, double *d_data, pu_ctx *ctx);
+int  pu_nc_format(pu_ncfile *f, int *version);
+int  pu_nc_close(pu_ncfile *f);
+
+/* The dycore's output fields (the write hook registered at dynamics_rk.h:1192-1233): with the
+ * coupler state converted as convert_coupler_to_dynamics does, density_pert = rho - hy_dens_cells,
+ * theta_pert = rho theta / rho - hy_theta_cells, pressure_pert = C0 (rho theta)^gamma -
+ * hy_pressure_cells (device outputs [nz][ny][nx]). */
+int  pu_dycore_output_fields(pu_ctx *ctx, const pu_coupler_state *cs, double *density_pert, double *theta_pert,
+                             double *pressure_pert);
+
+/* The dycore's restart hook (dynamics_rk.h:1234-1241): hydrostatic cell profiles from a restart
+ * file ([nz+2hs] each, host) replace the context's, the hydrostatic edges are recomputed
+ * (compute_hydrostasis_edges) and the immersed halos rebuilt from the coupler's
+ * immersed_proportion (create_immersed_proportion_halos). */
+int  pu_dycore_restart(pu_ctx *ctx, const pu_coupler_state *cs, const double *dens_c, const double *theta_c,
+                       const double *pres_c);
+
 /* FP64 reductions on the device: total mass sum(rho)*dV (rho = dry + mass-adding tracers), max
  * wind speed and max |w| (the quantities of Coupler::inform_user, coupler.h:507-530). */
 int  pu_diagnostics(pu_ctx *ctx, const pu_coupler_state *cs, double *mass, double *max_wind,
@@ -206,6 +248,14 @@ typedef struct {
   unsigned long pg_update_cycles;
 } pu_driver_forcing;
 int  pu_driver_set_forcing(pu_driver *d, const pu_driver_forcing *f);
+/* Coupler::write_output_file(prefix) (coupler.h:533-668): writes <prefix>_<counter>.nc (name
+ * returned in path_out) and advances the file counter; double_precision 0 writes the state as
+ * float like the reference (option output_precision).  pu_driver_restart sets option
+ * restart_file and runs Coupler::overwrite_with_restart (coupler.h:673-723). */
+int  pu_driver_write_output(pu_driver *d, const char *prefix, int double_precision, char *path_out, int path_len);
+int  pu_driver_restart(pu_driver *d, const char *path);
+/* option elapsed_time of the driver's coupler (s) */
+double pu_driver_elapsed_time(pu_driver *d);
 int  pu_driver_step(pu_driver *d, int nsteps);
 int  pu_driver_state(pu_driver *d, pu_coupler_state *out, int *num_tracers);
 pu_ctx *pu_driver_context(pu_driver *d);

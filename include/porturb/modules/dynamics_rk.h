@@ -57,21 +57,80 @@ struct Dynamics_Euler_Stratified_WenoFV {
     };
     put_u8("tracer_adds_mass", mass);
     put_u8("tracer_positive", pos);
-    int nz = coupler.get_nz();
-    int hsr = (coupler.get_option<int>("porturb_ord", ord) - 1) / 2;
+    publish_hydrostatics(coupler, ctx.ctx);
+    // output and restart (dynamics_rk.h:1190-1241)
+    coupler.register_output_variable<real>("immersed_proportion", core::Coupler::DIMS_3D);
+    coupler.register_write_output_module([](core::Coupler &c, core::NcFile &nc) { write_output(c, nc); });
+    coupler.register_overwrite_with_restart_module([](core::Coupler &c, core::NcFile &nc) { read_restart(c, nc); });
+  }
+
+  static int halo_of(core::Coupler const &c) { return (c.get_option<int>("porturb_ord", ord) - 1) / 2; }
+
+  // hy_{dens,theta,pressure}_{cells,edges} entries from the context's profiles
+  static void publish_hydrostatics(core::Coupler &coupler, pu_ctx *ctx) {
+    auto &dm = coupler.get_data_manager_readwrite();
+    int nz = coupler.get_nz(), hsr = halo_of(coupler);
     std::vector<double> c[3], e[3];
     for (int w = 0; w < 3; w++) { c[w].resize(nz + 2 * hsr); e[w].resize(nz + 1); }
-    detail::check(pu_dycore_hydrostatics(ctx.ctx, c[0].data(), c[1].data(), c[2].data(), e[0].data(),
-                                         e[1].data(), e[2].data()));
-    const char *cn[3] = {"hy_dens_cells", "hy_theta_cells", "hy_pressure_cells"};
-    const char *en[3] = {"hy_dens_edges", "hy_theta_edges", "hy_pressure_edges"};
+    detail::check(pu_dycore_hydrostatics(ctx, c[0].data(), c[1].data(), c[2].data(), e[0].data(), e[1].data(),
+                                         e[2].data()));
     for (int w = 0; w < 3; w++) {
-      if (!dm.entry_exists(cn[w])) dm.register_and_allocate<double>(cn[w], "", {(long long)c[w].size()});
-      if (!dm.entry_exists(en[w])) dm.register_and_allocate<double>(en[w], "", {(long long)e[w].size()});
-      detail::check(pu_memcpy_h2d(dm.get<double, 1>(cn[w]).data(), c[w].data(), c[w].size() * sizeof(double)));
-      detail::check(pu_memcpy_h2d(dm.get<double, 1>(en[w]).data(), e[w].data(), e[w].size() * sizeof(double)));
+      if (!dm.entry_exists(cell_names[w])) dm.register_and_allocate<double>(cell_names[w], "", {(long long)c[w].size()});
+      if (!dm.entry_exists(edge_names[w])) dm.register_and_allocate<double>(edge_names[w], "", {(long long)e[w].size()});
+      detail::check(pu_memcpy_h2d(dm.get<double, 1>(cell_names[w]).data(), c[w].data(), c[w].size() * sizeof(double)));
+      detail::check(pu_memcpy_h2d(dm.get<double, 1>(edge_names[w]).data(), e[w].data(), e[w].size() * sizeof(double)));
     }
   }
+
+  // output hook (dynamics_rk.h:1192-1233): hy_*_cells on z_halo, density/theta/pressure_pert
+  static void write_output(core::Coupler &c, core::NcFile &nc) {
+    auto &ctx = detail::get_context(c, ord);
+    const int nx = c.get_nx(), ny = c.get_ny(), nz = c.get_nz();
+    nc.redef();
+    nc.create_dim("z_halo", nz + 2 * halo_of(c));
+    for (int w = 0; w < 3; w++) {
+      if (c.output_double()) nc.create_var<double>(cell_names[w], {"z_halo"});
+      else nc.create_var<float>(cell_names[w], {"z_halo"});
+    }
+    for (const char *n : {"theta_pert", "pressure_pert", "density_pert"}) nc.create_var<real>(n, {"z", "y", "x"});
+    nc.enddef();
+    auto &dm = c.get_data_manager_readonly();
+    for (int w = 0; w < 3; w++) nc.write_all(dm.get<double const, 1>(cell_names[w]), cell_names[w]);
+    const long long n3 = (long long)nx * ny * nz;
+    void *buf = nullptr;
+    detail::check(pu_alloc_device(3 * (size_t)n3 * sizeof(double), &buf));
+    double *p = (double *)buf;
+    pu_coupler_state cs = c.coupler_state();
+    int rc = pu_dycore_output_fields(ctx.ctx, &cs, p, p + n3, p + 2 * n3);
+    if (rc == PU_OK) {
+      const char *names[3] = {"density_pert", "theta_pert", "pressure_pert"};
+      for (int f = 0; f < 3; f++) {
+        core::View<const double, 3> v;
+        v.ptr = p + f * n3;
+        v.dims = {nz, ny, nx};
+        nc.write_all(v, names[f]);
+      }
+    }
+    pu_free_device(buf);
+    detail::check(rc);
+  }
+
+  // restart hook (dynamics_rk.h:1234-1241)
+  static void read_restart(core::Coupler &c, core::NcFile &nc) {
+    auto &ctx = detail::get_context(c, ord);
+    const int nzh = c.get_nz() + 2 * halo_of(c);
+    std::vector<double> h[3];
+    for (int w = 0; w < 3; w++) {
+      h[w].resize(nzh);
+      nc.read(h[w], cell_names[w]);
+    }
+    pu_coupler_state cs = c.coupler_state();
+    detail::check(pu_dycore_restart(ctx.ctx, &cs, h[0].data(), h[1].data(), h[2].data()));
+    publish_hydrostatics(c, ctx.ctx);
+  }
+
+  static constexpr const char *cell_names[3] = {"hy_dens_cells", "hy_theta_cells", "hy_pressure_cells"};
+  static constexpr const char *edge_names[3] = {"hy_dens_edges", "hy_theta_edges", "hy_pressure_edges"};
 
   void time_step(core::Coupler &coupler, real dt_phys) const {
     auto &ctx = detail::get_context(coupler, ord);

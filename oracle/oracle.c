@@ -816,6 +816,26 @@ int oc_dycore_tendencies(oc_dycore *d, const double *coupler, double *tend) {
   return 0;
 }
 
+/* The dycore's output fields (the write hook at dynamics_rk.h:1192-1233): density_pert =
+ * rho - hy_dens_cells, theta_pert = rho theta / rho - hy_theta_cells, pressure_pert =
+ * C0 (rho theta)^gamma - hy_pressure_cells, with the state from convert_coupler_to_dynamics. */
+int oc_dycore_output_fields(const oc_dycore *d, const double *cp, double *dp, double *tp, double *pp) {
+  const oc_config *c = &d->cfg;
+  int nx = c->nx, ny = c->ny, nz = c->nz, hs = d->hs;
+  size_t nh = (size_t)(nx + 2 * hs) * (ny + 2 * hs) * (nz + 2 * hs);
+  double *state = (double *)calloc(5 * nh, sizeof(double));
+  double *tracers = (double *)calloc((size_t)(c->num_tracers > 0 ? c->num_tracers : 1) * nh, sizeof(double));
+  convert_c2d(c, hs, cp, state, tracers);
+  for (int k = 0; k < nz; k++) for (int j = 0; j < ny; j++) for (int i = 0; i < nx; i++) {
+    double r = H3(state + idR * nh, hs + k, hs + j, hs + i), rt = H3(state + idT * nh, hs + k, hs + j, hs + i);
+    I3(dp, k, j, i) = r - d->hy_dens_cells[hs + k];
+    I3(tp, k, j, i) = rt / r - d->hy_theta_cells[hs + k];
+    I3(pp, k, j, i) = c->C0 * pow(rt, c->gamma_d) - d->hy_pressure_cells[hs + k];
+  }
+  free(state); free(tracers);
+  return 0;
+}
+
 double oc_total_mass(const oc_config *c, const double *cp) {
   int nx = c->nx, ny = c->ny, nz = c->nz, ntr = c->num_tracers;
   size_t n3 = (size_t)nx * ny * nz;

@@ -86,6 +86,10 @@ int  oc_dycore_hydrostatics(const oc_dycore *d, double *dens_c, double *theta_c,
 /* dycore_any_immersed10 [nz][ny][nx] and dycore_immersed_tau [nz][ny][nx] (first relax call). */
 int  oc_dycore_immersed_masks(oc_dycore *d, unsigned char *any10, int *tau);
 
+/* density_pert, theta_pert, pressure_pert [nz][ny][nx] of the dycore's output hook. */
+int  oc_dycore_output_fields(const oc_dycore *d, const double *coupler, double *density_pert, double *theta_pert,
+                             double *pressure_pert);
+
 /* LES_Closure::apply (les_closure.h:30-323), in place on the coupler state. */
 int  oc_les_apply(const oc_config *cfg, double *coupler, const double *immersed_prop, double dt_phys);
 

@@ -70,6 +70,8 @@ def lib():
         L.oc_dycore_hydrostatics.restype = I
         L.oc_dycore_immersed_masks.argtypes = [P, P, P]
         L.oc_dycore_immersed_masks.restype = I
+        L.oc_dycore_output_fields.argtypes = [P, P, P, P, P]
+        L.oc_dycore_output_fields.restype = I
         L.oc_les_apply.argtypes = [cp, P, P, D]
         L.oc_les_apply.restype = I
         L.oc_total_mass.argtypes = [cp, P]
@@ -171,6 +173,14 @@ class Dycore:
         arrs = [np.zeros(nzh) for _ in range(3)] + [np.zeros(nze) for _ in range(3)]
         lib().oc_dycore_hydrostatics(self.h, *[a.ctypes.data for a in arrs])
         return arrs
+
+    def output_fields(self, coupler):
+        """density_pert, theta_pert, pressure_pert of the dycore's output hook."""
+        c = np.ascontiguousarray(coupler, dtype=np.float64)
+        shape = (self.cfg["nz"], self.cfg["ny"], self.cfg["nx"])
+        out = [np.zeros(shape) for _ in range(3)]
+        assert lib().oc_dycore_output_fields(self.h, c.ctypes.data, *[a.ctypes.data for a in out]) == 0
+        return out
 
     def immersed_masks(self):
         shape = (self.cfg["nz"], self.cfg["ny"], self.cfg["nx"])

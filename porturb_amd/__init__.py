@@ -93,6 +93,9 @@ def lib():
         "pu_driver_create": (I, [I, I, I, I, D, I, I, C.c_ulonglong, I, C.c_char_p, C.POINTER(P)]),
         "pu_driver_step": (I, [P, I]),
         "pu_driver_set_forcing": (I, [P, P]),
+        "pu_driver_write_output": (I, [P, C.c_char_p, I, C.c_char_p, I]),
+        "pu_driver_restart": (I, [P, C.c_char_p]),
+        "pu_driver_elapsed_time": (D, [P]),
         "pu_driver_state": (I, [P, csp, C.POINTER(I)]),
         "pu_driver_context": (P, [P]),
         "pu_driver_tracer_name": (I, [P, I, C.c_char_p, I]),
@@ -121,6 +124,21 @@ def lib():
         "pu_geostrophic_wind_forcing": (I, [P, csp, D, D, D, D]),
         "pu_level_mean_wind": (I, [P, csp, I, I, P, P]),
         "pu_add_uniform_wind": (I, [P, csp, D, D]),
+        "pu_nc_create": (I, [C.c_char_p, I, C.POINTER(P)]),
+        "pu_nc_open": (I, [C.c_char_p, C.POINTER(P)]),
+        "pu_nc_def_dim": (I, [P, C.c_char_p, C.c_longlong]),
+        "pu_nc_def_var": (I, [P, C.c_char_p, I, I, C.POINTER(C.c_char_p)]),
+        "pu_nc_redef": (I, [P]),
+        "pu_nc_enddef": (I, [P]),
+        "pu_nc_inq_var": (I, [P, C.c_char_p, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong)]),
+        "pu_nc_put_host": (I, [P, C.c_char_p, P, I]),
+        "pu_nc_get_host": (I, [P, C.c_char_p, P, I]),
+        "pu_nc_put_device": (I, [P, C.c_char_p, P, P]),
+        "pu_nc_get_device": (I, [P, C.c_char_p, P, P]),
+        "pu_nc_format": (I, [P, C.POINTER(I)]),
+        "pu_nc_close": (I, [P]),
+        "pu_dycore_output_fields": (I, [P, csp, P, P, P]),
+        "pu_dycore_restart": (I, [P, csp, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -519,6 +537,73 @@ def bf16_to_float(a):
     return (np.asarray(a).astype(np.uint16).astype(np.uint32) << 16).view(np.float32)
 
 
+PU_NC_BYTE, PU_NC_CHAR, PU_NC_SHORT, PU_NC_INT, PU_NC_FLOAT, PU_NC_DOUBLE, PU_NC_UBYTE, PU_NC_INT64 = 1, 2, 3, 4, 5, 6, 7, 10
+PU_NC_AUTO, PU_NC_CDF2, PU_NC_CDF5 = 0, 2, 5
+_NC_HOST = {np.dtype(np.float64): PU_NC_DOUBLE, np.dtype(np.float32): PU_NC_FLOAT, np.dtype(np.int32): PU_NC_INT}
+
+
+class NcFile:
+    """NetCDF classic file through libporturb's reader/writer (pu_nc_*)."""
+
+    def __init__(self, path, mode="r", format=PU_NC_AUTO):
+        p = C.c_void_p()
+        if mode == "w":
+            check(lib().pu_nc_create(path.encode(), format, C.byref(p)))
+        else:
+            check(lib().pu_nc_open(path.encode(), C.byref(p)))
+        self.f = p.value
+
+    def def_dim(self, name, n):
+        check(lib().pu_nc_def_dim(self.f, name.encode(), n))
+
+    def def_var(self, name, nc_type, dims):
+        arr = (C.c_char_p * max(len(dims), 1))(*[d.encode() for d in dims])
+        check(lib().pu_nc_def_var(self.f, name.encode(), nc_type, len(dims), arr))
+
+    def redef(self):
+        check(lib().pu_nc_redef(self.f))
+
+    def enddef(self):
+        check(lib().pu_nc_enddef(self.f))
+
+    def inq(self, name):
+        t, nd, dims = C.c_int(), C.c_int(), (C.c_longlong * 8)()
+        check(lib().pu_nc_inq_var(self.f, name.encode(), C.byref(t), C.byref(nd), dims))
+        return t.value, tuple(dims[i] for i in range(nd.value))
+
+    def put(self, name, arr):
+        a = np.ascontiguousarray(arr)
+        check(lib().pu_nc_put_host(self.f, name.encode(), a.ctypes.data, _NC_HOST[a.dtype]))
+
+    def get(self, name, dtype=np.float64):
+        _, shape = self.inq(name)
+        a = np.empty(shape, dtype)
+        check(lib().pu_nc_get_host(self.f, name.encode(), a.ctypes.data, _NC_HOST[np.dtype(dtype)]))
+        return a
+
+    def put_device(self, name, dptr, ctx):
+        check(lib().pu_nc_put_device(self.f, name.encode(), dptr, ctx))
+
+    def get_device(self, name, dptr, ctx):
+        check(lib().pu_nc_get_device(self.f, name.encode(), dptr, ctx))
+
+    def version(self):
+        v = C.c_int()
+        check(lib().pu_nc_format(self.f, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.f:
+            f, self.f = self.f, None
+            check(lib().pu_nc_close(f))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class PuDriverForcing(C.Structure):
     _fields_ = [("geostrophic", C.c_int), ("lat_g", C.c_double), ("u_g", C.c_double), ("v_g", C.c_double),
                 ("sponge", C.c_int), ("sponge_dt_multiple", C.c_double), ("sponge_top_prop", C.c_double),
@@ -547,6 +632,21 @@ class Driver:
         f = PuDriverForcing(**kw)
         if lib().pu_driver_set_forcing(self.d, C.byref(f)) != PU_OK:
             raise ValueError("pu_driver_set_forcing: " + lib().pu_driver_last_error().decode())
+
+    def write_output(self, prefix, double_precision=True):
+        """Coupler::write_output_file; returns the file name."""
+        buf = C.create_string_buffer(4096)
+        if lib().pu_driver_write_output(self.d, prefix.encode(), int(double_precision), buf, 4096) != PU_OK:
+            raise RuntimeError("pu_driver_write_output: " + lib().pu_driver_last_error().decode())
+        return buf.value.decode()
+
+    def restart(self, path):
+        """Coupler::overwrite_with_restart from `path`."""
+        if lib().pu_driver_restart(self.d, path.encode()) != PU_OK:
+            raise RuntimeError("pu_driver_restart: " + lib().pu_driver_last_error().decode())
+
+    def elapsed_time(self):
+        return lib().pu_driver_elapsed_time(self.d)
 
     def step(self, n=1):
         if lib().pu_driver_step(self.d, n) != PU_OK:

@@ -144,6 +144,35 @@ int pu_driver_step(pu_driver *d, int nsteps) {
   }
 }
 
+int pu_driver_write_output(pu_driver *d, const char *prefix, int double_precision, char *path_out, int path_len) {
+  if (!d || !prefix) return PU_ERR_ARG;
+  try {
+    d->coupler.set_option<std::string>("output_precision", double_precision ? "double" : "float");
+    std::string name = d->coupler.write_output_file(prefix, false);
+    if (path_out && path_len > 0) snprintf(path_out, path_len, "%s", name.c_str());
+    return PU_OK;
+  } catch (std::exception const &e) {
+    g_driver_error = e.what();
+    return PU_ERR_STATE;
+  }
+}
+
+int pu_driver_restart(pu_driver *d, const char *path) {
+  if (!d || !path) return PU_ERR_ARG;
+  try {
+    d->coupler.set_option<std::string>("restart_file", path);
+    d->coupler.overwrite_with_restart();
+    return PU_OK;
+  } catch (std::exception const &e) {
+    g_driver_error = e.what();
+    return PU_ERR_STATE;
+  }
+}
+
+double pu_driver_elapsed_time(pu_driver *d) {
+  return d ? d->coupler.get_option<double>("elapsed_time", 0.) : 0.;
+}
+
 int pu_driver_state(pu_driver *d, pu_coupler_state *out, int *num_tracers) {
   if (!d || !out) return PU_ERR_ARG;
   *out = d->coupler.coupler_state();

@@ -193,6 +193,71 @@ int fill_device(double *ptr, double value, size_t count) {
   return PU_OK;
 }
 
+// Hydrostatic edges from the cell profiles (dynamics_rk.h:1145-1186: 4th-order interpolation in
+// log space for density and pressure, linear for theta; 2nd order below ord 5) and the upload of
+// cells and edges to the device.
+static int upload_hydrostatics(pu_ctx *ctx) {
+  const pu_config &c = ctx->cfg;
+  const int hs = ctx->g.hs, nz = ctx->g.nz;
+  hipStream_t s = ctx->stream;
+  const std::vector<double> &r = ctx->hy_cells[0], &t = ctx->hy_cells[1], &p = ctx->hy_cells[2];
+  std::vector<double> &de = ctx->hy_edges[0], &te = ctx->hy_edges[1], &pe = ctx->hy_edges[2];
+  de.assign(nz + 1, 0.); te.assign(nz + 1, 0.); pe.assign(nz + 1, 0.);
+  for (int k = 0; k <= nz; k++) {
+    if (c.ord < 5) {
+      de[k] = std::exp(0.5 * std::log(r[hs + k - 1]) + 0.5 * std::log(r[hs + k]));
+      te[k] = 0.5 * t[hs + k - 1] + 0.5 * t[hs + k];
+      pe[k] = std::exp(0.5 * std::log(p[hs + k - 1]) + 0.5 * std::log(p[hs + k]));
+    } else {
+      de[k] = std::exp(-1. / 12. * std::log(r[hs + k - 2]) + 7. / 12. * std::log(r[hs + k - 1]) +
+                       7. / 12. * std::log(r[hs + k]) - 1. / 12. * std::log(r[hs + k + 1]));
+      te[k] = -1. / 12. * t[hs + k - 2] + 7. / 12. * t[hs + k - 1] + 7. / 12. * t[hs + k] - 1. / 12. * t[hs + k + 1];
+      pe[k] = std::exp(-1. / 12. * std::log(p[hs + k - 2]) + 7. / 12. * std::log(p[hs + k - 1]) +
+                       7. / 12. * std::log(p[hs + k]) - 1. / 12. * std::log(p[hs + k + 1]));
+    }
+  }
+  std::vector<double> all;
+  for (int w = 0; w < 3; w++) all.insert(all.end(), ctx->hy_cells[w].begin(), ctx->hy_cells[w].end());
+  for (int w = 0; w < 3; w++) all.insert(all.end(), ctx->hy_edges[w].begin(), ctx->hy_edges[w].end());
+  PU_HIP_CHECK(hipMemcpyAsync(ctx->d_hy, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  PU_HIP_CHECK(hipStreamSynchronize(s));
+  return PU_OK;
+}
+
+// Immersed data from immersed_proportion: the flag array of the halo'd box
+// (dycore_immersed_proportion_halos > 0, dynamics_rk.h:1010-1030), tau and the relaxation
+// multiplier (:389-425), any_immersed10 (:1127-1143) and the closure's halo'd proportion
+// (sc_init.h:728-767).
+static int build_immersed(pu_ctx *ctx, const double *prop) {
+  const pu::Grid &g = ctx->g;
+  const pu_config &c = ctx->cfg;
+  hipStream_t s = ctx->stream;
+  long long n3 = g.n3();
+  ImmParams P;
+  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp;
+  P.nxh = g.nxh; P.nyh = g.nyh; P.nzh = g.nzh; P.n3 = n3; P.power = c.immersed_power;
+  long long nh = g.nh();
+  hipLaunchKernelGGL(k_imm_flag, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, prop, ctx->d_imm_flag, P);
+  PU_HIP_CHECK(hipGetLastError());
+  dim3 grd((unsigned)((n3 + 255) / 256)), blk(256);
+  hipLaunchKernelGGL(k_tau_relax, grd, blk, 0, s, prop, ctx->d_tau, ctx->d_relax, P);
+  PU_HIP_CHECK(hipGetLastError());
+  uint8_t *tmp1 = nullptr, *tmp2 = nullptr;
+  PU_HIP_CHECK(hipMalloc(&tmp1, n3));
+  PU_HIP_CHECK(hipMalloc(&tmp2, n3));
+  hipLaunchKernelGGL(k_any_x, grd, blk, 0, s, prop, tmp1, P, 10);
+  hipLaunchKernelGGL(k_any_y, grd, blk, 0, s, tmp1, tmp2, P, 10);
+  hipLaunchKernelGGL(k_any_z, grd, blk, 0, s, tmp2, ctx->d_any10, P, 10);
+  PU_HIP_CHECK(hipGetLastError());
+  long long nl = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+  hipLaunchKernelGGL(k_imm_les, dim3((unsigned)((nl + 255) / 256)), blk, 0, s, prop, ctx->d_imm_les_val,
+                     g.nx, g.ny, g.nz);
+  PU_HIP_CHECK(hipGetLastError());
+  PU_HIP_CHECK(hipStreamSynchronize(s));
+  hipFree(tmp1); hipFree(tmp2);
+  return PU_OK;
+}
+
 int dycore_init_device(pu_ctx *ctx, const pu_coupler_state *cs) {
   const pu::Grid &g = ctx->g;
   const pu_config &c = ctx->cfg;
@@ -236,55 +301,10 @@ int dycore_init_device(pu_ctx *ctx, const pu_coupler_state *cs) {
         p[k] = C0 * std::pow(r[k] * theta0, gamma);
       }
     }
-    // hydrostatic edges (dynamics_rk.h:1145-1186)
-    std::vector<double> &de = ctx->hy_edges[0], &te = ctx->hy_edges[1], &pe = ctx->hy_edges[2];
-    de.assign(nz + 1, 0.); te.assign(nz + 1, 0.); pe.assign(nz + 1, 0.);
-    for (int k = 0; k <= nz; k++) {
-      if (c.ord < 5) {
-        de[k] = std::exp(0.5 * std::log(r[hs + k - 1]) + 0.5 * std::log(r[hs + k]));
-        te[k] = 0.5 * t[hs + k - 1] + 0.5 * t[hs + k];
-        pe[k] = std::exp(0.5 * std::log(p[hs + k - 1]) + 0.5 * std::log(p[hs + k]));
-      } else {
-        de[k] = std::exp(-1. / 12. * std::log(r[hs + k - 2]) + 7. / 12. * std::log(r[hs + k - 1]) +
-                         7. / 12. * std::log(r[hs + k]) - 1. / 12. * std::log(r[hs + k + 1]));
-        te[k] = -1. / 12. * t[hs + k - 2] + 7. / 12. * t[hs + k - 1] + 7. / 12. * t[hs + k] - 1. / 12. * t[hs + k + 1];
-        pe[k] = std::exp(-1. / 12. * std::log(p[hs + k - 2]) + 7. / 12. * std::log(p[hs + k - 1]) +
-                         7. / 12. * std::log(p[hs + k]) - 1. / 12. * std::log(p[hs + k + 1]));
-      }
-    }
-    std::vector<double> all;
-    for (int w = 0; w < 3; w++) all.insert(all.end(), ctx->hy_cells[w].begin(), ctx->hy_cells[w].end());
-    for (int w = 0; w < 3; w++) all.insert(all.end(), ctx->hy_edges[w].begin(), ctx->hy_edges[w].end());
-    PU_HIP_CHECK(hipMemcpyAsync(ctx->d_hy, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    PU_HIP_CHECK(hipStreamSynchronize(s));
+    int rc = upload_hydrostatics(ctx);
+    if (rc != PU_OK) return rc;
   }
-  // immersed data
-  {
-    ImmParams P;
-    P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp;
-    P.nxh = g.nxh; P.nyh = g.nyh; P.nzh = g.nzh; P.n3 = n3; P.power = c.immersed_power;
-    const double *prop = cs->immersed_proportion;
-    long long nh = g.nh();
-    hipLaunchKernelGGL(k_imm_flag, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, prop, ctx->d_imm_flag, P);
-    PU_HIP_CHECK(hipGetLastError());
-    dim3 grd((unsigned)((n3 + 255) / 256)), blk(256);
-    hipLaunchKernelGGL(k_tau_relax, grd, blk, 0, s, prop, ctx->d_tau, ctx->d_relax, P);
-    PU_HIP_CHECK(hipGetLastError());
-    uint8_t *tmp1 = nullptr, *tmp2 = nullptr;
-    PU_HIP_CHECK(hipMalloc(&tmp1, n3));
-    PU_HIP_CHECK(hipMalloc(&tmp2, n3));
-    hipLaunchKernelGGL(k_any_x, grd, blk, 0, s, prop, tmp1, P, 10);
-    hipLaunchKernelGGL(k_any_y, grd, blk, 0, s, tmp1, tmp2, P, 10);
-    hipLaunchKernelGGL(k_any_z, grd, blk, 0, s, tmp2, ctx->d_any10, P, 10);
-    PU_HIP_CHECK(hipGetLastError());
-    long long nl = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
-    hipLaunchKernelGGL(k_imm_les, dim3((unsigned)((nl + 255) / 256)), blk, 0, s, prop, ctx->d_imm_les_val,
-                       g.nx, g.ny, g.nz);
-    PU_HIP_CHECK(hipGetLastError());
-    PU_HIP_CHECK(hipStreamSynchronize(s));
-    hipFree(tmp1); hipFree(tmp2);
-  }
-  return PU_OK;
+  return build_immersed(ctx, cs->immersed_proportion);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -475,3 +495,80 @@ int case_init_device(const pu_config *cfg, int case_id, unsigned long long seed,
 }
 
 }  // namespace pu
+
+// ------------------------------------------------------------------------------------------------
+// Output fields and restart of the dycore (the output / restart hooks of dynamics_rk.h:1192-1241)
+// ------------------------------------------------------------------------------------------------
+namespace pu {
+
+struct OutParams {
+  int nx, ny, nz, hs, ntr, idWV;
+  long long n3;
+  double R_d, R_v, C0, gamma;
+  unsigned int adds_mass_mask;
+  const double *hy;       // cells: dens [nz+2hs], theta [nz+2hs], pressure [nz+2hs]
+};
+
+// density_pert, theta_pert, pressure_pert from the coupler state converted as
+// convert_coupler_to_dynamics does (dynamics_rk.h:1304-1352, 1213-1231)
+__global__ __launch_bounds__(256) void k_dycore_output(const double *__restrict__ rd_, const double *__restrict__ tt,
+                                                       TracerPtrsC trc, double *dp, double *tp, double *pp, OutParams P) {
+  const long long id = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (id >= P.n3) return;
+  const int k = (int)(id / ((long long)P.nx * P.ny));
+  const double rho_d = rd_[id], temp = tt[id];
+  const double rho_v = trc.p[P.idWV][id];
+  const double press = rho_d * P.R_d * temp + rho_v * P.R_v * temp;
+  double rho = rho_d;
+  for (int tr = 0; tr < P.ntr; tr++) if ((P.adds_mass_mask >> tr) & 1u) rho += trc.p[tr][id];
+  const double theta = pow(press / P.C0, 1. / P.gamma) / rho;
+  const double rt = rho * theta;
+  const int nzh = P.nz + 2 * P.hs;
+  dp[id] = rho - P.hy[P.hs + k];
+  tp[id] = rt / rho - P.hy[nzh + P.hs + k];
+  pp[id] = P.C0 * pow(rt, P.gamma) - P.hy[2 * nzh + P.hs + k];
+}
+
+}  // namespace pu
+
+using namespace pu;
+
+extern "C" {
+
+int pu_dycore_output_fields(pu_ctx *ctx, const pu_coupler_state *cs, double *density_pert, double *theta_pert,
+                            double *pressure_pert) {
+  if (!ctx || !cs || !cs->density_dry || !cs->temp || !density_pert || !theta_pert || !pressure_pert)
+    return fail(PU_ERR_ARG, "pu_dycore_output_fields: null argument");
+  if (!ctx->dycore_ready) return fail(PU_ERR_STATE, "pu_dycore_init has not been called");
+  const pu_config &c = ctx->cfg;
+  OutParams P;
+  P.nx = c.nx; P.ny = c.ny; P.nz = c.nz; P.hs = ctx->g.hs; P.ntr = c.num_tracers; P.idWV = c.idWV;
+  P.n3 = ctx->g.n3();
+  P.R_d = c.R_d; P.R_v = c.R_v; P.C0 = c.C0; P.gamma = c.gamma_d;
+  P.adds_mass_mask = 0;
+  for (int tr = 0; tr < c.num_tracers; tr++) if (c.tracer_adds_mass[tr]) P.adds_mass_mask |= 1u << tr;
+  P.hy = ctx->d_hy;
+  TracerPtrsC tp;
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++) tp.p[tr] = tr < c.num_tracers ? cs->tracers[tr] : nullptr;
+  hipLaunchKernelGGL(k_dycore_output, dim3((unsigned)((P.n3 + 255) / 256)), dim3(256), 0, ctx->stream, cs->density_dry,
+                     cs->temp, tp, density_pert, theta_pert, pressure_pert, P);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
+int pu_dycore_restart(pu_ctx *ctx, const pu_coupler_state *cs, const double *dens_c, const double *theta_c,
+                      const double *pres_c) {
+  if (!ctx || !cs || !cs->immersed_proportion || !dens_c || !theta_c || !pres_c)
+    return fail(PU_ERR_ARG, "pu_dycore_restart: null argument");
+  if (!ctx->dycore_ready) return fail(PU_ERR_STATE, "pu_dycore_init has not been called");
+  const int nzh = ctx->g.nz + 2 * ctx->g.hs;
+  ctx->hy_cells[0].assign(dens_c, dens_c + nzh);
+  ctx->hy_cells[1].assign(theta_c, theta_c + nzh);
+  ctx->hy_cells[2].assign(pres_c, pres_c + nzh);
+  int rc = upload_hydrostatics(ctx);
+  if (rc != PU_OK) return rc;
+  ctx->sgs_valid_ready = false;          // the learned closure's mask follows the immersed cells
+  return build_immersed(ctx, cs->immersed_proportion);
+}
+
+}  // extern "C"
