@@ -9,11 +9,14 @@ apply_surface_fluxes + one LES_Closure::apply, inputs resident in HBM.  Timed wi
 the library's stream.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2]
-                    [--config abl_fp64|abl_fp32|abl_fp32_learned|bubble]
+                    [--config abl_fp64|abl_fp32|abl_fp32_learned|bubble|windfarm]
 
 abl_fp32_learned: the FP32 dycore with the learned subgrid closure (network file
 porturb_amd/data/sgs_abl_urban_dx2.bin, trained by porturb_amd.sgs_training) in place of the
 eddy-viscosity momentum/heat fluxes, the TKE closure as its fallback.
+windfarm: 2048x1024x256 cells at 5 m, precursor x inflow, capping inversion, FP32 + learned
+closure, 3x4 actuator-disk turbines (D = 120 m, hub 90 m, C_T = 0.75,
+porturb_amd/data/turbine_disk_ct075_d120.yaml) after the surface fluxes.
 """
 import argparse
 import json
@@ -26,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/sec per SSPRK3 step, 1 MI355X; % HBM roofline; err vs FP64"
 SGS_MODEL = os.path.join(ROOT, "porturb_amd", "data", "sgs_abl_urban_dx2.bin")
+TURBINE_FILE = os.path.join(ROOT, "porturb_amd", "data", "turbine_disk_ct075_d120.yaml")
 
 
 def make_workload(name):
@@ -36,6 +40,8 @@ def make_workload(name):
         return cases.abl_urban(precision=PU_FP32), True
     if name == "bubble":
         return cases.bubble(n=64), False
+    if name == "windfarm":
+        return cases.windfarm(precision=PU_FP32), True
     raise SystemExit("unknown config " + name)
 
 
@@ -100,11 +106,14 @@ def main():
     print("bench: %s, %d cells, warmup %d, steps %d" % (cfg["name"], n3, args.warmup, args.steps),
           file=sys.stderr, flush=True)
     # the drop-in C++ path: core::Coupler + Dynamics_Euler_Stratified_WenoFV (+ LES_Closure)
-    learned = args.config == "abl_fp32_learned"
+    learned = args.config in ("abl_fp32_learned", "windfarm")
+    turbines = args.config == "windfarm"
     if learned and not os.path.exists(SGS_MODEL):
         raise SystemExit("bench: %s missing (train it with python -m porturb_amd.sgs_training)" % SGS_MODEL)
     d = pa.Driver(cfg["case_id"], cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlen"] / cfg["nx"], ord=cfg["ord"],
                   precision=cfg["precision"], seed=1234, closure=les, sgs_model=SGS_MODEL if learned else None)
+    if turbines:
+        d.set_turbines(TURBINE_FILE, 4, 3, orig_C_T=True)
 
     def step():
         d.step(1)
@@ -126,7 +135,7 @@ def main():
     for _ in range(2):
         step()
     d.synchronize()
-    fams = ["weno_x", "weno_y", "weno_z", "prep", "surface", "les", "sgs"]
+    fams = ["weno_x", "weno_y", "weno_z", "prep", "surface", "les", "sgs", "turbines"]
     times = {f: d.kernel_time(f) for f in fams}
     d.kernel_timing(False)
     dom = max(["weno_x", "weno_y", "weno_z"], key=lambda f: times[f][0])
@@ -157,11 +166,13 @@ def main():
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64" if fp64 else "f32", "data": "synthetic",
-           "config": {"workload": cfg["name"] + ("_learned" if learned else ""), "cells": n3, "ord": cfg["ord"],
+           "config": {"workload": cfg["name"] + ("_learned" if learned else "") + ("_turbines3x4" if turbines else ""),
+                      "cells": n3, "ord": cfg["ord"],
                       "path": "C++ Coupler + modules",
                       "closure": ("learned (bf16 matrix cores) + tke fallback" if learned else "tke") if les else "none",
                       "tracers": cfg["num_tracers"],
-                      "step": "dycore.time_step + apply_surface_fluxes + les_closure.apply" if les else "dycore.time_step"},
+                      "step": ("dycore.time_step + apply_surface_fluxes + " +
+                               ("windmills.apply + " if turbines else "") + "les_closure.apply") if les else "dycore.time_step"},
            "roofline": roof}
     d.destroy()
     if not args.no_cpu_baseline:

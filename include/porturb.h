@@ -146,6 +146,49 @@ int  pu_level_mean_wind(pu_ctx *ctx, const pu_coupler_state *cs, int k_begin, in
 int  pu_add_uniform_wind(pu_ctx *ctx, pu_coupler_state *cs, double du, double dv);
 
 /* ------------------------------------------------------------------------------------------
+ * modules::WindmillActuators (model/modules/windmill_actuators_yaw.h:12-1065) as actuator disks:
+ * the disk projection and upstream sampling, table-driven thrust / power coefficients at the
+ * inertially filtered inflow, thrust and TKE tendencies with the normal-wind limiter, yaw
+ * controller; no blades, floating motions or immersed hub/tower (the reference's wind-farm drivers
+ * run without them, windfarm_ensemble.cpp:106-110).  The turbine table is the reference's turbine
+ * file (RefTurbine::init, :34-79): velocity_magnitude, thrust_coef, power_coef, power_megawatts,
+ * hub_height, blade_radius, max_yaw_speed (rad/s), overhang, hub_radius, shaft_tilt (rad). */
+typedef struct {
+  int    ntable;
+  const double *velocity_magnitude, *thrust_coef, *power_coef, *power_megawatts;   /* host, ntable each */
+  double hub_height, blade_radius, max_yaw_speed, overhang, hub_radius, shaft_tilt;
+} pu_turbine_ref;
+/* options turbine_initial_yaw (rad), turbine_fixed_yaw, turbine_orig_C_T, turbine_upstream_dir */
+typedef struct {
+  double initial_yaw;
+  int    fixed_yaw, orig_C_T, have_upstream_dir;
+  double upstream_dir;
+} pu_turbine_opts;
+/* one turbine after its last apply: yaw (rad), inertial inflow, table values, sampled inflow
+ * sum(w u), sum(w v), disk-cube sums of u and v, instantaneous normal inflow, number of applies */
+typedef struct {
+  double yaw, u_samp_inertial, v_samp_inertial, power, ct, cp, u_samp, v_samp, upstream_u, upstream_v, instant_mag0;
+  int    nsteps;
+} pu_turbine_status;
+/* time traces kept per apply (the reference's power/yaw/u_samp/v_samp/cp/ct traces, :111-125) */
+enum { PU_TURB_POWER = 0, PU_TURB_YAW = 1, PU_TURB_U_SAMP = 2, PU_TURB_V_SAMP = 3, PU_TURB_CP = 4, PU_TURB_CT = 5,
+       PU_TURB_NTRACE = 6 };
+typedef struct pu_turbines pu_turbines;
+/* WindmillActuators::init's turbine placement done by the caller: tower bases x[i], y[i] (m);
+ * apply_thrust NULL = all (option turbine_apply_thrust).  Needs the TKE tracer (idTKE). */
+int  pu_turbines_create(pu_ctx *ctx, const pu_turbine_ref *ref, int nturb, const double *x, const double *y,
+                        const unsigned char *apply_thrust, const pu_turbine_opts *opts, pu_turbines **out);
+/* WindmillActuators::apply (:523-1043): uvel, vvel and the TKE tracer in place; proj_weight_tot /
+ * samp_weight_tot (device [nz][ny][nx], or NULL) receive windmill_proj_weight /
+ * windmill_samp_weight.  Synchronises the stream once per turbine (table lookups on the host). */
+int  pu_turbines_apply(pu_turbines *t, pu_coupler_state *cs, double dt, double *proj_weight_tot,
+                       double *samp_weight_tot);
+int  pu_turbines_status(pu_turbines *t, int iturb, pu_turbine_status *out);
+/* trace `which` (PU_TURB_*) of turbine iturb: status.nsteps values (the yaw in radians) */
+int  pu_turbines_trace(pu_turbines *t, int iturb, int which, double *out);
+int  pu_turbines_destroy(pu_turbines *t);
+
+/* ------------------------------------------------------------------------------------------
  * Output / restart files: NetCDF classic with fixed-size variables, the format the reference's
  * Coupler::write_output_file / overwrite_with_restart (model/core/coupler.h:533-723) write and
  * read through yakl::SimplePNetCDF (create_dim, create_var, enddef, redef, write, write_all,
@@ -254,6 +297,12 @@ int  pu_driver_set_forcing(pu_driver *d, const pu_driver_forcing *f);
  * restart_file and runs Coupler::overwrite_with_restart (coupler.h:673-723). */
 int  pu_driver_write_output(pu_driver *d, const char *prefix, int double_precision, char *path_out, int path_len);
 int  pu_driver_restart(pu_driver *d, const char *path);
+/* Actuator-disk turbines in the driver step (after the surface fluxes, before the closure, as
+ * windfarm_ensemble.cpp:226-229): modules::WindmillActuators with option turbine_file, a farm of
+ * ncols_x x nrows_y turbines at x = (i+1) xlen/(ncols_x+1), y = (j+1/2) ylen/nrows_y, fixed yaw 0,
+ * upstream direction 0, no blades; orig_C_T sets option turbine_orig_C_T.  Needs the closure. */
+int  pu_driver_set_turbines(pu_driver *d, const char *turbine_file, int ncols_x, int nrows_y, int orig_C_T);
+pu_turbines *pu_driver_turbines(pu_driver *d);
 /* option elapsed_time of the driver's coupler (s) */
 double pu_driver_elapsed_time(pu_driver *d);
 int  pu_driver_step(pu_driver *d, int nsteps);

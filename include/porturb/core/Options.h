@@ -13,6 +13,7 @@
 #include <string>
 #include <type_traits>
 #include <variant>
+#include <vector>
 
 namespace core {
 
@@ -26,7 +27,8 @@ struct PortUrbError : std::runtime_error {
 
 class Options {
  public:
-  using Value = std::variant<bool, int, long long, unsigned long, float, double, std::string>;
+  using Value = std::variant<bool, int, long long, unsigned long, float, double, std::string, std::vector<double>,
+                             std::vector<bool>>;
 
   template <class T> void add_option(std::string key, T value) {
     if (option_exists(key)) endrun("ERROR: Options::add_option: key [" + key + "] already exists");

@@ -148,6 +148,27 @@ int    oc_uniform_pg_forcing(const oc_config *c, double *coupler, double dt, dou
 int    oc_uniform_pg_forcing_height(const oc_config *c, double *coupler, double dt, double height, double u0,
                                     double v0, double tau);
 
+/* modules::WindmillActuators::apply, actuator disks (turbine.c) */
+typedef struct {
+  int ntab;
+  const double *velmag, *thrust_coef, *power_coef, *power_mw;
+  double hub_height, blade_radius, max_yaw_speed, overhang, hub_radius, shaft_tilt;
+} oc_turbine_ref;
+typedef struct {
+  double base_x, base_y, yaw, u_inertial, v_inertial, yaw_uavg, yaw_vavg;
+  int apply_thrust;
+  double power, ct, cp, u_samp, v_samp, upstream_u, upstream_v;   /* of the last apply */
+} oc_turbine;
+typedef struct {
+  int fixed_yaw, orig_C_T, have_upstream_dir;
+  double upstream_dir;
+} oc_turbine_opts;
+double oc_turbine_interp(const double *ref_umag, const double *ref_var, int n, double umag);
+double oc_turbine_blade_weight(double dx, double rad);
+double oc_turbine_yaw_tend(oc_turbine *t, double uvel, double vvel, double dt, double max_yaw_speed);
+int    oc_turbines_apply(const oc_config *c, double *coupler, const oc_turbine_ref *ref, oc_turbine *turbs, int nturb,
+                         const oc_turbine_opts *o, double dt, double *proj_weight_tot, double *samp_weight_tot);
+
 /* Sum over cells of the dycore's full density rho (dry + mass-adding tracers), times cell volume. */
 double oc_total_mass(const oc_config *cfg, const double *coupler);
 

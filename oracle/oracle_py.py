@@ -101,6 +101,15 @@ def lib():
         L.oc_les_apply_learned.restype = I
         L.oc_surface_fluxes.argtypes = [cp, P, P, P, P, P, P, D]
         L.oc_surface_fluxes.restype = I
+        L.oc_turbine_interp.argtypes = [P, P, I, D]
+        L.oc_turbine_interp.restype = D
+        L.oc_turbine_blade_weight.argtypes = [D, D]
+        L.oc_turbine_blade_weight.restype = D
+        L.oc_turbine_yaw_tend.argtypes = [C.POINTER(OcTurbine), D, D, D, D]
+        L.oc_turbine_yaw_tend.restype = D
+        L.oc_turbines_apply.argtypes = [cp, P, C.POINTER(OcTurbineRef), C.POINTER(OcTurbine), I,
+                                        C.POINTER(OcTurbineOpts), D, P, P]
+        L.oc_turbines_apply.restype = I
         L.oc_sponge_layer.argtypes = [cp, P, D, D, D]
         L.oc_sponge_layer.restype = I
         L.oc_geostrophic_forcing.argtypes = [cp, P, D, D, D, D]
@@ -317,6 +326,48 @@ def les_apply_learned(cfg, coupler, imm, dt, model, bf16=True, nthreads=0):
     assert lib().oc_les_apply_learned(C.byref(oc), coupler.ctypes.data, imm.ctypes.data, dt, C.byref(model.s),
                                       int(bf16), tau.ctypes.data, flags.ctypes.data) == 0
     return tau, flags
+
+
+class OcTurbineRef(C.Structure):
+    _fields_ = [("ntab", C.c_int), ("velmag", C.c_void_p), ("thrust_coef", C.c_void_p), ("power_coef", C.c_void_p),
+                ("power_mw", C.c_void_p), ("hub_height", C.c_double), ("blade_radius", C.c_double),
+                ("max_yaw_speed", C.c_double), ("overhang", C.c_double), ("hub_radius", C.c_double),
+                ("shaft_tilt", C.c_double)]
+
+
+class OcTurbine(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("base_x", "base_y", "yaw", "u_inertial", "v_inertial", "yaw_uavg",
+                                          "yaw_vavg")] + [("apply_thrust", C.c_int)] + \
+               [(n, C.c_double) for n in ("power", "ct", "cp", "u_samp", "v_samp", "upstream_u", "upstream_v")]
+
+
+class OcTurbineOpts(C.Structure):
+    _fields_ = [("fixed_yaw", C.c_int), ("orig_C_T", C.c_int), ("have_upstream_dir", C.c_int),
+                ("upstream_dir", C.c_double)]
+
+
+class TurbineRef:
+    """A reference turbine: table arrays (kept alive here) plus geometry."""
+
+    def __init__(self, velmag, thrust_coef, power_coef, power_mw, hub_height, blade_radius, max_yaw_speed=0.5 / 180 * np.pi,
+                 overhang=None, hub_radius=None, shaft_tilt=0.0):
+        self.arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in (velmag, thrust_coef, power_coef, power_mw)]
+        self.s = OcTurbineRef(len(self.arrays[0]), *[a.ctypes.data for a in self.arrays], hub_height, blade_radius,
+                              max_yaw_speed, -0.1 * blade_radius if overhang is None else overhang,
+                              0.03 * blade_radius if hub_radius is None else hub_radius, shaft_tilt)
+
+
+def turbines_apply(cfg, coupler, ref, turbs, dt, fixed_yaw=False, orig_C_T=False, upstream_dir=None):
+    """modules::WindmillActuators::apply (actuator disks) in place; turbs is a ctypes array of
+    OcTurbine (state carried between calls).  Returns (proj_weight_tot, samp_weight_tot)."""
+    oc = to_oc_config(cfg)
+    assert coupler.flags["C_CONTIGUOUS"] and coupler.dtype == np.float64
+    shape = (cfg["nz"], cfg["ny"], cfg["nx"])
+    pw, sw = np.zeros(shape), np.zeros(shape)
+    o = OcTurbineOpts(int(fixed_yaw), int(orig_C_T), int(upstream_dir is not None), upstream_dir or 0.0)
+    assert lib().oc_turbines_apply(C.byref(oc), coupler.ctypes.data, C.byref(ref.s), turbs, len(turbs), C.byref(o), dt,
+                                   pw.ctypes.data, sw.ctypes.data) == 0
+    return pw, sw
 
 
 class OcPgState(C.Structure):

@@ -96,6 +96,8 @@ def lib():
         "pu_driver_write_output": (I, [P, C.c_char_p, I, C.c_char_p, I]),
         "pu_driver_restart": (I, [P, C.c_char_p]),
         "pu_driver_elapsed_time": (D, [P]),
+        "pu_driver_set_turbines": (I, [P, C.c_char_p, I, I, I]),
+        "pu_driver_turbines": (P, [P]),
         "pu_driver_state": (I, [P, csp, C.POINTER(I)]),
         "pu_driver_context": (P, [P]),
         "pu_driver_tracer_name": (I, [P, I, C.c_char_p, I]),
@@ -137,6 +139,11 @@ def lib():
         "pu_nc_get_device": (I, [P, C.c_char_p, P, P]),
         "pu_nc_format": (I, [P, C.POINTER(I)]),
         "pu_nc_close": (I, [P]),
+        "pu_turbines_create": (I, [P, P, I, P, P, P, P, C.POINTER(P)]),
+        "pu_turbines_apply": (I, [P, csp, D, P, P]),
+        "pu_turbines_status": (I, [P, I, P]),
+        "pu_turbines_trace": (I, [P, I, I, P]),
+        "pu_turbines_destroy": (I, [P]),
         "pu_dycore_output_fields": (I, [P, csp, P, P, P]),
         "pu_dycore_restart": (I, [P, csp, P, P, P]),
     }
@@ -604,6 +611,70 @@ class NcFile:
         self.close()
 
 
+class PuTurbineRef(C.Structure):
+    _fields_ = [("ntable", C.c_int), ("velocity_magnitude", C.c_void_p), ("thrust_coef", C.c_void_p),
+                ("power_coef", C.c_void_p), ("power_megawatts", C.c_void_p), ("hub_height", C.c_double),
+                ("blade_radius", C.c_double), ("max_yaw_speed", C.c_double), ("overhang", C.c_double),
+                ("hub_radius", C.c_double), ("shaft_tilt", C.c_double)]
+
+
+class PuTurbineOpts(C.Structure):
+    _fields_ = [("initial_yaw", C.c_double), ("fixed_yaw", C.c_int), ("orig_C_T", C.c_int),
+                ("have_upstream_dir", C.c_int), ("upstream_dir", C.c_double)]
+
+
+class PuTurbineStatus(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("yaw", "u_samp_inertial", "v_samp_inertial", "power", "ct", "cp", "u_samp",
+                                          "v_samp", "upstream_u", "upstream_v", "instant_mag0")] + [("nsteps", C.c_int)]
+
+
+PU_TURB_POWER, PU_TURB_YAW, PU_TURB_U_SAMP, PU_TURB_V_SAMP, PU_TURB_CP, PU_TURB_CT = range(6)
+
+
+class Turbines:
+    """Actuator-disk turbines (pu_turbines_*) on a Dycore context."""
+
+    def __init__(self, dycore, velmag, thrust_coef, power_coef, power_mw, hub_height, blade_radius, x, y,
+                 max_yaw_speed=0.5 / 180 * np.pi, overhang=None, hub_radius=None, shaft_tilt=0.0, apply_thrust=None,
+                 initial_yaw=0.0, fixed_yaw=False, orig_C_T=False, upstream_dir=None):
+        self.tab = [np.ascontiguousarray(a, dtype=np.float64) for a in (velmag, thrust_coef, power_coef, power_mw)]
+        ref = PuTurbineRef(len(self.tab[0]), *[a.ctypes.data for a in self.tab], hub_height, blade_radius,
+                           max_yaw_speed, -0.1 * blade_radius if overhang is None else overhang,
+                           0.03 * blade_radius if hub_radius is None else hub_radius, shaft_tilt)
+        opts = PuTurbineOpts(initial_yaw, int(fixed_yaw), int(orig_C_T), int(upstream_dir is not None), upstream_dir or 0.0)
+        xs, ys = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64)
+        at = None if apply_thrust is None else np.ascontiguousarray(apply_thrust, np.uint8)
+        p = C.c_void_p()
+        check(lib().pu_turbines_create(dycore.ctx, C.byref(ref), len(xs), xs.ctypes.data, ys.ctypes.data,
+                                       None if at is None else at.ctypes.data, C.byref(opts), C.byref(p)))
+        self.t, self.n = p.value, len(xs)
+
+    def apply(self, cs, dt, proj=None, samp=None):
+        check(lib().pu_turbines_apply(self.t, C.byref(cs.c), dt, proj.ptr if proj else None, samp.ptr if samp else None))
+
+    def status(self, i):
+        st = PuTurbineStatus()
+        check(lib().pu_turbines_status(self.t, i, C.byref(st)))
+        return st
+
+    def trace(self, i, which):
+        n = self.status(i).nsteps
+        out = np.zeros(n)
+        check(lib().pu_turbines_trace(self.t, i, which, out.ctypes.data))
+        return out
+
+    def destroy(self):
+        if self.t:
+            lib().pu_turbines_destroy(self.t)
+            self.t = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class PuDriverForcing(C.Structure):
     _fields_ = [("geostrophic", C.c_int), ("lat_g", C.c_double), ("u_g", C.c_double), ("v_g", C.c_double),
                 ("sponge", C.c_int), ("sponge_dt_multiple", C.c_double), ("sponge_top_prop", C.c_double),
@@ -647,6 +718,16 @@ class Driver:
 
     def elapsed_time(self):
         return lib().pu_driver_elapsed_time(self.d)
+
+    def set_turbines(self, turbine_file, ncols_x, nrows_y, orig_C_T=False):
+        """modules::WindmillActuators (actuator disks) in the step: a ncols_x x nrows_y farm."""
+        if lib().pu_driver_set_turbines(self.d, turbine_file.encode(), ncols_x, nrows_y, int(orig_C_T)) != PU_OK:
+            raise RuntimeError("pu_driver_set_turbines: " + lib().pu_driver_last_error().decode())
+
+    def turbine_status(self, i):
+        st = PuTurbineStatus()
+        check(lib().pu_turbines_status(lib().pu_driver_turbines(self.d), i, C.byref(st)))
+        return st
 
     def step(self, n=1):
         if lib().pu_driver_step(self.d, n) != PU_OK:

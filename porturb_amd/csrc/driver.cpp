@@ -5,9 +5,10 @@
 // the coupled state, add tracers, initialise the case, init the LES closure and the dycore, then
 // per step, with dt = dycore.compute_time_step: the optional column forcing
 // (geostrophic_wind_forcing, uniform_pg_wind_forcing[_height]), run_module(dycore.time_step), the
-// optional sponge_layer, run_module(apply_surface_fluxes) for the rough-wall ABL cases and
-// run_module(les_closure.apply) -- the reference's module order (abl_neutral.cpp:98-103,
-// city.cpp:130-141) without its time averager.  Exposed through pu_driver_* for bench.py and the
+// optional sponge_layer, run_module(apply_surface_fluxes) for the rough-wall ABL cases, the
+// optional actuator-disk turbines and run_module(les_closure.apply) -- the reference's module
+// order (abl_neutral.cpp:98-103, city.cpp:130-141, windfarm_ensemble.cpp:226-229) without its
+// time averager.  Exposed through pu_driver_* for bench.py and the
 // tests.
 #include <cmath>
 #include <memory>
@@ -21,6 +22,7 @@
 #include "porturb/modules/sponge_layer.h"
 #include "porturb/modules/surface_flux.h"
 #include "porturb/modules/uniform_pg_wind_forcing.h"
+#include "porturb/modules/windmill_actuators.h"
 
 struct pu_driver {
   core::Coupler coupler;
@@ -29,6 +31,8 @@ struct pu_driver {
   bool closure = true;
   bool surface = false;     // rough-wall surface fluxes (the ABL cases)
   pu_driver_forcing forcing{};
+  modules::WindmillActuators windmills;
+  bool turbines = false;
   std::string error;
 };
 
@@ -112,6 +116,36 @@ int pu_driver_set_forcing(pu_driver *d, const pu_driver_forcing *f) {
   return PU_OK;
 }
 
+int pu_driver_set_turbines(pu_driver *d, const char *turbine_file, int ncols_x, int nrows_y, int orig_C_T) {
+  if (!d || !turbine_file || ncols_x < 1 || nrows_y < 1) return PU_ERR_ARG;
+  try {
+    core::Coupler &c = d->coupler;
+    if (!d->closure) core::endrun("ERROR: turbines need the LES closure (TKE)");
+    std::vector<double> xs, ys;
+    for (int j = 0; j < nrows_y; j++)
+      for (int i = 0; i < ncols_x; i++) {
+        xs.push_back((i + 1) * c.get_xlen() / (ncols_x + 1));
+        ys.push_back((j + 0.5) * c.get_ylen() / nrows_y);
+      }
+    c.set_option<std::string>("turbine_file", turbine_file);
+    c.set_option<std::vector<double>>("turbine_x_locs", xs);
+    c.set_option<std::vector<double>>("turbine_y_locs", ys);
+    c.set_option<bool>("turbine_do_blades", false);
+    c.set_option<bool>("turbine_fixed_yaw", true);
+    c.set_option<double>("turbine_initial_yaw", 0.);
+    c.set_option<double>("turbine_upstream_dir", 0.);
+    c.set_option<bool>("turbine_orig_C_T", orig_C_T != 0);
+    d->windmills.init(c);
+    d->turbines = true;
+    return PU_OK;
+  } catch (std::exception const &e) {
+    g_driver_error = e.what();
+    return PU_ERR_STATE;
+  }
+}
+
+pu_turbines *pu_driver_turbines(pu_driver *d) { return d && d->turbines ? d->windmills.handle() : nullptr; }
+
 int pu_driver_step(pu_driver *d, int nsteps) {
   if (!d) return PU_ERR_ARG;
   try {
@@ -134,6 +168,7 @@ int pu_driver_step(pu_driver *d, int nsteps) {
         c.run_module([&](core::Coupler &cc) { modules::sponge_layer(cc, dt, f.sponge_dt_multiple * dt, f.sponge_top_prop); },
                      "sponge");
       if (d->surface) c.run_module([&](core::Coupler &cc) { modules::apply_surface_fluxes(cc, dt); }, "surface_fluxes");
+      if (d->turbines) c.run_module([&](core::Coupler &cc) { d->windmills.apply(cc, dt); }, "windmillactuators");
       if (d->closure) c.run_module([&](core::Coupler &cc) { d->les.apply(cc, dt); }, "les_closure");
       c.set_option<double>("elapsed_time", c.get_option<double>("elapsed_time") + dt);
     }
