@@ -95,7 +95,9 @@ int pu_create(const pu_config *cfg, void *stream, pu_ctx **out) {
   g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
   g.hs = (cfg->ord - 1) / 2;
   g.hp = prim_halo(g.hs);
-  g.nxh = g.nx + 2 * g.hp; g.nyh = g.ny + 2 * g.hp; g.nzh = g.nz + 2 * g.hp;
+  g.ox = PRIM_OX;
+  g.nxh = (g.ox + g.nx + g.hp + PRIM_ROW_ALIGN - 1) / PRIM_ROW_ALIGN * PRIM_ROW_ALIGN;
+  g.nyh = g.ny + 2 * g.hp; g.nzh = g.nz + 2 * g.hp;
   c->nv = NUM_STATE + cfg->num_tracers;
   c->nf = c->nv + 1;
   size_t n3 = (size_t)g.n3(), nh = (size_t)g.nh();

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_hydro_means(const double *rd_, const do
 // Immersed precompute
 // ------------------------------------------------------------------------------------------------
 struct ImmParams {
-  int nx, ny, nz, hs, hp;
+  int nx, ny, nz, hs, hp, ox;
   long long nxh, nyh, nzh, n3;
   double power;
 };
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_imm_flag(const double *prop, uint8_t *f
   long long nh = P.nxh * P.nyh * P.nzh;
   if (id >= nh) return;
   int ih = (int)(id % P.nxh), jh = (int)((id / P.nxh) % P.nyh), kh = (int)(id / (P.nxh * P.nyh));
-  int k = kh - P.hp, j = jh - P.hp, i = ih - P.hp;
+  int k = kh - P.hp, j = jh - P.hp, i = ih - P.ox;
   double v;
   if (k < 0 || k >= P.nz) v = 1.;
   else v = prop[((long long)k * P.ny + wrapi_(j, P.ny)) * P.nx + wrapi_(i, P.nx)];
@@ -234,7 +234,7 @@ static int build_immersed(pu_ctx *ctx, const double *prop) {
   hipStream_t s = ctx->stream;
   long long n3 = g.n3();
   ImmParams P;
-  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp;
+  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ox = g.ox;
   P.nxh = g.nxh; P.nyh = g.nyh; P.nzh = g.nzh; P.n3 = n3; P.power = c.immersed_power;
   long long nh = g.nh();
   hipLaunchKernelGGL(k_imm_flag, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s, prop, ctx->d_imm_flag, P);

@@ -95,8 +95,9 @@ constexpr int LBX = LTX + 2, LBY = LTY + 2, LBZ = LTZ + 2;   // tile + one-cell 
 template <class T> struct TileBox {
   const T *sm;
   int nfl;                                   // fields before the immersed-proportion slot
-  const float *tau;                          // learned closure: [9][n] stresses/heat fluxes, or null
-  const unsigned char *flag;                 // learned closure: [n] flags
+  const float *tau;                          // learned closure: global [9][n3] stresses/heat fluxes, or null
+  long long n3;
+  const unsigned char *flag;                 // learned closure: [n] flags of the box points
   static constexpr int sx = 1, sy = LBX, sz = LBX * LBY, n = LBX * LBY * LBZ;
   __device__ T S(int f, int idx) const { return sm[f * n + idx]; }
   __device__ T imm(int idx) const { return sm[nfl * n + idx]; }
@@ -105,9 +106,11 @@ template <class T> struct TileBox {
 // Face flux on the low face of box cell c (at halo coordinates kh, jh, ih of the closure box) in
 // direction DIR (les_closure.h:92-229).  T = double reproduces the reference's formulas in FP64;
 // T = float evaluates them in FP32 (the FP32 dycore mode).
+// gm, gc: the global cell ids of the face's low and high cells (periodic boundaries wrapped), used
+// by the learned closure only.
 template <int DIR, class T>
 __device__ __forceinline__ void face_flux(const TileBox<T> &B, const LesParams &P, int c, int kh, int jh, int ih,
-                                          FaceCommon<T> &fc) {
+                                          long long gm, long long gc, FaceCommon<T> &fc) {
   constexpr int sx = TileBox<T>::sx, sy = TileBox<T>::sy, sz = TileBox<T>::sz;
   const int sn = DIR == 0 ? sx : (DIR == 1 ? sy : sz);            // normal stride
   const int m = c - sn;                                            // the low neighbour
@@ -191,17 +194,17 @@ __device__ __forceinline__ void face_flux(const TileBox<T> &B, const LesParams &
     // Learned closure (no reference counterpart): on a face whose two cells both hold the
     // network's flag, the momentum fluxes become rho * avg(tau_{i,DIR}) (bit 0) and the theta
     // flux rho * avg(q_DIR) (bit 1).  Box points outside the domain carry flag 0 unless the
-    // boundary is periodic (then they hold the wrapped cell's values).
+    // boundary is periodic (then they hold the wrapped cell's values).  The stresses are read from
+    // the network's output array directly (neighbouring threads read neighbouring cells).
     const int both = B.flag[m] & B.flag[c];
     if (both & 1) {
       for (int f = 0; f < 3; f++) {
         const int lo_ = f < DIR ? f : DIR, hi_ = f < DIR ? DIR : f;
         const int ti = lo_ == 0 ? hi_ : (lo_ == 1 ? 2 + hi_ : 5);
-        fc.f[f] = rho * (h * ((T)B.tau[ti * TileBox<T>::n + m] + (T)B.tau[ti * TileBox<T>::n + c]));
+        fc.f[f] = rho * (h * ((T)B.tau[ti * B.n3 + gm] + (T)B.tau[ti * B.n3 + gc]));
       }
     }
-    if (both & 2)
-      fc.f[3] = rho * (h * ((T)B.tau[(6 + DIR) * TileBox<T>::n + m] + (T)B.tau[(6 + DIR) * TileBox<T>::n + c]));
+    if (both & 2) fc.f[3] = rho * (h * ((T)B.tau[(6 + DIR) * B.n3 + gm] + (T)B.tau[(6 + DIR) * B.n3 + gc]));
   }
 }
 
@@ -254,14 +257,11 @@ __global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, dou
         if (f0 + df <= nfl && q < NB) sm[(f0 + df) * NB + q] = buf[df][e];
       }
   }
-  float *stau = nullptr;
   unsigned char *sflag = nullptr;
   if (P.sflags) {
-    // the network's outputs and flags of the box points (interior cells; periodic boundaries wrap)
-    stau = reinterpret_cast<float *>(sm + NB * (nfl + 1));
-    sflag = reinterpret_cast<unsigned char *>(stau + 9 * NB);
-    long long cid[PER];
-    bool cok[PER];
+    // the network's flags of the box points (interior cells; periodic boundaries wrap)
+    sflag = reinterpret_cast<unsigned char *>(sm + NB * (nfl + 1));
+    unsigned char fb[PER];
 #pragma unroll
     for (int e = 0; e < PER; e++) {
       const int q = threadIdx.x + 256 * e;
@@ -270,25 +270,12 @@ __global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, dou
       if (i < 0 || i >= P.nx) { if (P.bc_x == PU_BC_PERIODIC) i = wrapl(i, P.nx); else ok = false; }
       if (j < 0 || j >= P.ny) { if (P.bc_y == PU_BC_PERIODIC) j = wrapl(j, P.ny); else ok = false; }
       if (k < 0 || k >= P.nz) { if (P.bc_z == PU_BC_PERIODIC) k = wrapl(k, P.nz); else ok = false; }
-      cok[e] = ok;
-      cid[e] = ok ? ((long long)k * P.ny + j) * P.nx + i : 0;
-    }
-    float tb[9][PER];
-    unsigned char fb[PER];
-#pragma unroll
-    for (int e = 0; e < PER; e++) {
-      fb[e] = cok[e] ? P.sflags[cid[e]] : (unsigned char)0;
-#pragma unroll
-      for (int o = 0; o < 9; o++) tb[o][e] = cok[e] ? P.tau[o * P.n3 + cid[e]] : 0.f;
+      fb[e] = ok ? P.sflags[((long long)k * P.ny + j) * P.nx + i] : (unsigned char)0;
     }
 #pragma unroll
     for (int e = 0; e < PER; e++) {
       const int q = threadIdx.x + 256 * e;
-      if (q < NB) {
-        sflag[q] = fb[e];
-#pragma unroll
-        for (int o = 0; o < 9; o++) stau[o * NB + q] = tb[o][e];
-      }
+      if (q < NB) sflag[q] = fb[e];
     }
   }
   __syncthreads();
@@ -297,17 +284,24 @@ __global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, dou
   if (i >= P.nx || j >= P.ny || k >= P.nz) return;
   const long long id = ((long long)k * P.ny + j) * P.nx + i;
   const int ih = i + 1, jh = j + 1, kh = k + 1;
-  const TileBox<T> B{sm, nfl, stau, sflag};
+  const TileBox<T> B{sm, nfl, P.sflags ? P.tau : nullptr, P.n3, sflag};
   constexpr int sx = TileBox<T>::sx, sy = TileBox<T>::sy, sz = TileBox<T>::sz;
   const int c = ((lk + 1) * LBY + (lj + 1)) * LBX + (li + 1);
   auto S = [&](int f, int idx) { return B.S(f, idx); };
   FaceCommon<T> fx0, fx1, fy0, fy1, fz0, fz1;
-  face_flux<0>(B, P, c, kh, jh, ih, fx0);
-  face_flux<0>(B, P, c + sx, kh, jh, ih + 1, fx1);
-  face_flux<1>(B, P, c, kh, jh, ih, fy0);
-  face_flux<1>(B, P, c + sy, kh, jh + 1, ih, fy1);
-  face_flux<2>(B, P, c, kh, jh, ih, fz0);
-  face_flux<2>(B, P, c + sz, kh + 1, jh, ih, fz1);
+  // global ids of the six neighbours (wrapped where periodic; out-of-domain ones carry flag 0)
+  auto gid = [&](int ii, int jj, int kk) {
+    ii = P.bc_x == PU_BC_PERIODIC ? wrapl(ii, P.nx) : ii;
+    jj = P.bc_y == PU_BC_PERIODIC ? wrapl(jj, P.ny) : jj;
+    kk = P.bc_z == PU_BC_PERIODIC ? wrapl(kk, P.nz) : kk;
+    return ((long long)kk * P.ny + jj) * P.nx + ii;
+  };
+  face_flux<0>(B, P, c, kh, jh, ih, gid(i - 1, j, k), id, fx0);
+  face_flux<0>(B, P, c + sx, kh, jh, ih + 1, id, gid(i + 1, j, k), fx1);
+  face_flux<1>(B, P, c, kh, jh, ih, gid(i, j - 1, k), id, fy0);
+  face_flux<1>(B, P, c + sy, kh, jh + 1, ih, id, gid(i, j + 1, k), fy1);
+  face_flux<2>(B, P, c, kh, jh, ih, gid(i, j, k - 1), id, fz0);
+  face_flux<2>(B, P, c + sz, kh + 1, jh, ih, id, gid(i, j, k + 1), fz1);
   const T h = T(0.5);
   const T rdx = (T)P.rdx, rdy = (T)P.rdy, rdz = (T)P.rdz;
   const T r2dx = h * rdx, r2dy = h * rdy, r2dz = h * rdz;
@@ -430,7 +424,7 @@ template <class T>
 static int launch_les_update(pu_ctx *ctx, pu_coupler_state *cs, const LesParams &P, const TracerPtrs &tp) {
   const dim3 grd((unsigned)((P.nx + LTX - 1) / LTX), (unsigned)((P.ny + LTY - 1) / LTY), (unsigned)((P.nz + LTZ - 1) / LTZ));
   // box fields (<= 119 KB for 16 tracers in FP64) + the network's outputs and flags (24 KB)
-  const size_t lds = sizeof(T) * TileBox<T>::n * (7 + P.nq) + (P.sflags ? TileBox<T>::n * (9 * sizeof(float) + 1) : 0);
+  const size_t lds = sizeof(T) * TileBox<T>::n * (7 + P.nq) + (P.sflags ? TileBox<T>::n : 0);
   if (lds > 65536)
     PU_HIP_CHECK(hipFuncSetAttribute((const void *)k_les_update<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(k_les_update<T>, grd, dim3(256), lds, ctx->stream, (const T *)ctx->d_les, cs->uvel, cs->vvel,

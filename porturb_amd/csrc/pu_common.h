@@ -20,9 +20,16 @@ constexpr int idR = 0, idU = 1, idV = 2, idW = 3, idT = 4, idP = 5, NUM_STATE = 
 // plus one more for the neighbour-cell reconstruction each line segment needs at its ends.
 __host__ __device__ constexpr int prim_halo(int hs) { return hs + 1; }
 
+// x position of interior cell 0 in the primitive-field and flag arrays, and the alignment of their
+// rows (elements): rows start on 128-byte boundaries for float and double and the interior of
+// every row starts on one, so the row segments the kernels read and write map onto whole cache
+// lines.  PRIM_OX >= prim_halo(hs) for every order.
+constexpr int PRIM_OX = 32, PRIM_ROW_ALIGN = 32;
+
 struct Grid {
   int nx, ny, nz, hs, hp;          // hp = prim_halo(hs)
-  long long nxh, nyh, nzh;         // halo'd extents of the primitive-field array
+  int ox;                          // x offset of the interior (PRIM_OX); y/z offsets are hp
+  long long nxh, nyh, nzh;         // extents of the primitive-field array (nxh: padded row pitch)
   long long n3() const { return (long long)nx * ny * nz; }
   long long nh() const { return nxh * nyh * nzh; }
 };
