@@ -20,14 +20,14 @@
 namespace pu {
 
 struct PrepParams {
-  int nx, ny, nz, hs, hp, ntr, bc_x, bc_y, bc_z;
+  int nx, ny, nz, hs, hp, ox, ntr, bc_x, bc_y, bc_z;
   long long n3, nxh, nyh, nzh, nh;
   double C0, gamma;
   const double *hy_r, *hy_t, *hy_p;     // cells [nz+2hs]
 };
 
 struct SweepParams {
-  int nx, ny, nz, hs, hp, ntr, bc_along, wall_z, weno_all;
+  int nx, ny, nz, hs, hp, ox, ntr, bc_along, wall_z, weno_all;
   long long n3, nxh, nyh, nzh, nh;
   double rd;                       // 1/dx, 1/dy or 1/dz
   const double *hy_rc, *hy_tc;     // cells [nz+2hs]
@@ -79,7 +79,8 @@ __global__ __launch_bounds__(256) void k_prep(const ST *__restrict__ U, const ST
   int jh = blockIdx.y * blockDim.y + threadIdx.y;
   int kh = blockIdx.z;
   if (ih >= P.nxh || jh >= P.nyh || kh >= P.nzh) return;
-  int i = ih - P.hp, j = jh - P.hp, k = kh - P.hp;
+  int i = ih - P.ox, j = jh - P.hp, k = kh - P.hp;
+  if (i < -P.hp || i >= P.nx + P.hp) return;          // row padding
   bool ox = i < 0 || i >= P.nx, oy = j < 0 || j >= P.ny, oz = k < 0 || k >= P.nz;
   if ((int)ox + (int)oy + (int)oz >= 2) return;
   int is = i, js = j, ks = k;
@@ -397,7 +398,7 @@ __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep
   const bool wallz = DIR == 2 && P.wall_z;
 
   const long long sstride = DIR == 0 ? 1 : (DIR == 1 ? P.nxh : P.nxh * P.nyh);
-  const long long base = ((long long)(k + P.hp) * P.nyh + (j + P.hp)) * P.nxh + (i + P.hp);
+  const long long base = ((long long)(k + P.hp) * P.nyh + (j + P.hp)) * P.nxh + (i + P.ox);
 
   // immersed flags of the stencil -> zero-derivative clamp window, imm_L/imm_R, do_map
   int lo = 0, hi = ORD - 1;
@@ -660,7 +661,7 @@ template <class ST>
 static int prep_t(pu_ctx *ctx, const void *U, const void *Ulo, hipStream_t s) {
   const pu::Grid &g = ctx->g;
   PrepParams P;
-  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ntr = ctx->cfg.num_tracers;
+  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ox = g.ox; P.ntr = ctx->cfg.num_tracers;
   P.bc_x = ctx->cfg.bc_x; P.bc_y = ctx->cfg.bc_y; P.bc_z = ctx->cfg.bc_z;
   P.n3 = g.n3(); P.nxh = g.nxh; P.nyh = g.nyh; P.nzh = g.nzh; P.nh = g.nh();
   P.C0 = ctx->cfg.C0; P.gamma = ctx->cfg.gamma_d;
@@ -683,7 +684,7 @@ template <int ORD, class T, int DIR>
 static int sweep_one(pu_ctx *ctx, hipStream_t s, int mode, int stage, double dt, int nrelax) {
   const pu::Grid &g = ctx->g;
   SweepParams P;
-  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ntr = ctx->cfg.num_tracers;
+  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ox = g.ox; P.ntr = ctx->cfg.num_tracers;
   P.bc_along = DIR == 0 ? ctx->cfg.bc_x : (DIR == 1 ? ctx->cfg.bc_y : ctx->cfg.bc_z);
   if (DIR == 2 && P.bc_along == PU_BC_SOLID_WALL) P.bc_along = PU_BC_PRECURSOR;   // non-periodic
   P.wall_z = ctx->cfg.bc_z == PU_BC_SOLID_WALL;
