@@ -244,16 +244,41 @@ __device__ __forceinline__ void weno_edges(const T *s, bool do_map, bool imm_L, 
   for (int m = 0; m < S; m++) { wl[m] = T(O::idealL(m)) * rinv[m]; wr[m] = T(O::idealR(m)) * rinv[m]; }
   convexify<T, S>(wl);
   convexify<T, S>(wr);
-  if (do_map) {
+  if (do_map && ORD <= 9) {
+    // map_rs(w, d) = A/B; the renormalised mapped weights are A_m Q_m / sum_k A_k Q_k with
+    // Q_m = prod_{j != m} B_j, and the one division goes to the face value.  Safe in FP32 up to
+    // ord 9: the weights are normalised, so B_m = (w-d)^6 + 2000 (w(1-w))^3 lies in
+    // [0.99 d_m^6, 32], and with the ideal weights of ord <= 9 (>= 1/126) a product of S-1 factors
+    // stays above 1e-29 and below 1e6 (ord 11 could reach 1e-42: it keeps the divisions below).
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      T *w = side == 0 ? wl : wr;
+      T A[S], B[S], Q[S];
+#pragma unroll
+      for (int m = 0; m < S; m++) {
+        const T d = T(side == 0 ? O::idealL(m) : O::idealR(m));
+        const T x = w[m];
+        if constexpr (O::use_map_rs) {
+          const T dl = x - d, d2 = dl * dl, t1 = d2 * d2 * d2, ww = fma(-x, x, x), t2 = ww * ww * ww;
+          B[m] = fma(T(2000), t2, t1);
+          A[m] = fma(d, B[m], t1 * dl);
+        } else {
+          B[m] = d * d + (T(1) - T(2) * d) * x;
+          A[m] = x * (d + d * d - T(3) * d * x + x * x);
+        }
+      }
+      excl_products<T, S>(B, Q);
+      T sum = T(0);
+#pragma unroll
+      for (int m = 0; m < S; m++) { w[m] = A[m] * Q[m]; sum += w[m]; }
+      if (side == 0) qL = O::template combL<T>(s, w) * recip(sum);
+      else qR = O::template combR<T>(s, w) * recip(sum);
+    }
+  } else if (do_map) {
 #pragma unroll
     for (int m = 0; m < S; m++) {
-      if constexpr (O::use_map_rs) {
-        wl[m] = map_rs(wl[m], T(O::idealL(m)));
-        wr[m] = map_rs(wr[m], T(O::idealR(m)));
-      } else {
-        wl[m] = map_w(wl[m], T(O::idealL(m)));
-        wr[m] = map_w(wr[m], T(O::idealR(m)));
-      }
+      wl[m] = map_rs(wl[m], T(O::idealL(m)));
+      wr[m] = map_rs(wr[m], T(O::idealR(m)));
     }
     // renormalisation folded into the face value (the mapped weights sum to about one)
     T sl = wl[0], sr = wr[0];
