@@ -44,6 +44,10 @@ CASES = [
     dict(ord=5, nx=64, ny=12, nz=16, bc_z=pa.PU_BC_PERIODIC),
     dict(ord=9, nx=24, ny=20, nz=16, bc_z=pa.PU_BC_PERIODIC),
     dict(ord=11, nx=24, ny=20, nz=20),
+    # whole-column z sweeps with 2 and 4 cells per thread, the last thread partly filled
+    dict(ord=5, nx=12, ny=10, nz=51),
+    dict(ord=9, nx=16, ny=10, nz=101),
+    dict(ord=7, nx=12, ny=10, nz=128),
 ]
 
 
@@ -74,9 +78,9 @@ def test_hydrostatics_and_immersed_masks_match(ord_):
     assert (ga == oa).all() and (gt == ot).all()
 
 
-@pytest.mark.parametrize("ord_,steps,les", [(5, 10, False), (9, 4, True)])
-def test_time_steps_match_oracle(ord_, steps, les):
-    cfg = small_config(nx=24, ny=20, nz=18, ord=ord_, tracers=[WV, TKE, P1, P2])
+@pytest.mark.parametrize("ord_,steps,les,nz", [(5, 10, False, 18), (9, 4, True, 18), (9, 3, True, 70)])
+def test_time_steps_match_oracle(ord_, steps, les, nz):
+    cfg = small_config(nx=24, ny=20, nz=nz, ord=ord_, tracers=[WV, TKE, P1, P2])
     st, imm = random_state(cfg, seed=33, buildings=3)
     d, cs = gpu_setup(cfg, st, imm)
     od = oracle_py.Dycore(cfg, st, imm)
