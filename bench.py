@@ -157,7 +157,13 @@ def main():
         fam = json.load(open(tfile)).get("families", {}).get(dom)
         if fam:
             traffic = fam["read"] + fam["write"]
-    roof = {"bound": "valu", "kernel": "k_sweep<%d,%s,%s>" % (cfg["ord"], "double" if fp64 else "float", dom[-1]),
+    tname = "double" if fp64 else "float"
+    if dom == "weno_z" and cfg["bc_z"] == pa.PU_BC_SOLID_WALL:
+        kname = "k_sweep_zwall<%d, %s>" % (cfg["ord"], tname)          # whole-column z sweep
+    else:
+        kname = "k_sweep<%d, %s, %d, %d>" % (cfg["ord"], tname, {"weno_x": 0, "weno_y": 1, "weno_z": 2}[dom],
+                                             {"weno_x": 2, "weno_y": 1, "weno_z": 0}[dom])
+    roof = {"bound": "valu", "kernel": kname,
             "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic,
             "flops_per_launch": flops_launch, "avg_launch_ms": round(avg_ms, 4),

@@ -293,6 +293,54 @@ __device__ __forceinline__ void weno_edges(const T *s, bool do_map, bool imm_L, 
   }
 }
 
+// Immersed flags of a cell's stencil (dynamics_rk.h:572-607 with modify_stencil_immersed_der0,
+// :478-493): the zero-derivative clamp window [lo, hi] used for the transverse velocities and p,
+// imm_L / imm_R, do_map = no immersed cell within the 10-cell box, and WENO or the linear
+// reconstruction.
+template <int ORD> struct StencilFlags {
+  static constexpr int hs = (ORD - 1) / 2;
+  int lo = 0, hi = ORD - 1;
+  bool imm_L = false, imm_R = false, do_map = true, use_weno = true;
+
+  // fbase: flag-array index of the cell; any: its any-immersed-within-10 flag
+  __device__ __forceinline__ void setup(const uint8_t *__restrict__ immf, long long fbase, long long sstride, bool any,
+                                        int weno_all) {
+    uint8_t f[ORD];
+#pragma unroll
+    for (int s = 0; s < ORD; s++) f[s] = immf[fbase + (long long)(s - hs) * sstride];
+    imm_L = f[hs - 1];
+    imm_R = f[hs + 1];
+    if (!f[hs]) {
+#pragma unroll
+      for (int s = ORD - 1; s > hs; s--) if (f[s]) hi = s - 1;
+#pragma unroll
+      for (int s = 0; s < hs; s++) if (f[s]) lo = s + 1;
+    }
+    do_map = !any;
+    use_weno = any || weno_all;
+  }
+
+  // both interface values of the cell whose field value sits at p (stride sstride along the sweep)
+  template <class T>
+  __device__ __forceinline__ void recon(const T *p, long long sstride, bool modify, T &qL, T &qR) const {
+    p -= (long long)hs * sstride;
+    T s[ORD];
+    if (modify) {
+#pragma unroll
+      for (int q = 0; q < ORD; q++) s[q] = p[(long long)clampi(q, lo, hi) * sstride];
+    } else {
+#pragma unroll
+      for (int q = 0; q < ORD; q++) s[q] = p[(long long)q * sstride];
+    }
+    if (use_weno) {
+      weno_edges<ORD, T>(s, do_map, imm_L, imm_R, qL, qR);
+    } else {
+      qL = pu_weno::Ops<ORD>::template linL<T>(s);
+      qR = pu_weno::Ops<ORD>::template linR<T>(s);
+    }
+  }
+};
+
 // ------------------------------------------------------------------------------------------------
 // SSPRK3 stage combination of one conserved field of one cell (time_step_rk_3_3,
 // dynamics_rk.h:279-367): gravity and Coriolis sources, the stage's convex combination, the
@@ -425,50 +473,20 @@ __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep
   const long long sstride = DIR == 0 ? 1 : (DIR == 1 ? P.nxh : P.nxh * P.nyh);
   const long long base = ((long long)(k + P.hp) * P.nyh + (j + P.hp)) * P.nxh + (i + P.ox);
 
-  // immersed flags of the stencil -> zero-derivative clamp window, imm_L/imm_R, do_map
-  int lo = 0, hi = ORD - 1;
-  bool imm_L = false, imm_R = false, do_map = true, use_weno = true;
+  StencilFlags<ORD> sf;
   if (do_recon) {
     // A periodic neighbour cell past either end is the wrapped interior cell: its flags come from
     // that cell's own position (the z halos of the flag array hold "immersed", not a periodic copy,
     // dynamics_rk.h:1031-1034; the reference takes that face state from the wrapped cell's
     // reconstruction through edge_exchange, :890-908).
     const int cw = wrapi(c, nal);
-    const long long fbase = base + (long long)(cw - c) * sstride;
-    uint8_t f[ORD];
-#pragma unroll
-    for (int s = 0; s < ORD; s++) f[s] = immf[fbase + (long long)(s - hs) * sstride];
-    imm_L = f[hs - 1];
-    imm_R = f[hs + 1];
-    if (!f[hs]) {
-#pragma unroll
-      for (int s = ORD - 1; s > hs; s--) if (f[s]) hi = s - 1;
-#pragma unroll
-      for (int s = 0; s < hs; s++) if (f[s]) lo = s + 1;
-    }
     int ci = i, cj = j, ck = k;
     if (DIR == 0) ci = cw; else if (DIR == 1) cj = cw; else ck = cw;
-    bool any = any10[((long long)ck * P.ny + cj) * P.nx + ci] != 0;
-    do_map = !any;
-    use_weno = any || P.weno_all;
+    sf.setup(immf, base + (long long)(cw - c) * sstride, sstride, any10[((long long)ck * P.ny + cj) * P.nx + ci] != 0,
+             P.weno_all);
   }
-
   auto recon = [&](int l, bool modify, T &qL, T &qR) {
-    const T *p = prim + (long long)l * P.nh + base - (long long)hs * sstride;
-    T s[ORD];
-    if (modify) {
-#pragma unroll
-      for (int q = 0; q < ORD; q++) s[q] = p[(long long)clampi(q, lo, hi) * sstride];
-    } else {
-#pragma unroll
-      for (int q = 0; q < ORD; q++) s[q] = p[(long long)q * sstride];
-    }
-    if (use_weno) {
-      weno_edges<ORD, T>(s, do_map, imm_L, imm_R, qL, qR);
-    } else {
-      qL = pu_weno::Ops<ORD>::template linL<T>(s);
-      qR = pu_weno::Ops<ORD>::template linR<T>(s);
-    }
+    sf.template recon<T>(prim + (long long)l * P.nh + base, sstride, modify, qL, qR);
   };
 
   const long long cid = ((long long)k * P.ny + j) * P.nx + i;
@@ -572,6 +590,134 @@ __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep
     bufB0[lin] = F;
     __syncthreads();
     if (has_tend) accumulate(fidx, -(bufB0[lin + nbs] - bufB0[lin]) * rd, pf);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// z sweep between solid walls (the first sweep of a stage, writing the tendencies): one block per
+// 8 columns walks the whole column in chunks of 32 cells, every thread reconstructing one cell per
+// chunk.  Walls need no neighbour cells (the wall faces take the adjacent cell's own values,
+// dynamics_rk.h:864-888), and the face between two chunks takes the lower chunk's top values from
+// a small LDS carry, so no cell is reconstructed twice (the segment kernel reconstructs 160 cells
+// for each 128-level column).  Thread (x, t) of chunk q owns cell r = 32q + t: it computes the
+// face below r (r = 0: the bottom wall) and finishes the tendency of cell r-1; the thread of the
+// top cell also finishes its own tendency with the top wall face, whose Riemann state with w = 0
+// on both sides and equal left/right values is exactly mass flux 0 and pressure p (every other
+// flux through it is 0).  Otherwise the arithmetic of k_sweep<ORD, T, 2, 0>, operand for operand.
+// ------------------------------------------------------------------------------------------------
+constexpr int ZW_BX = 8, ZW_BS = 32;
+#ifndef PU_ZWALL
+#define PU_ZWALL 1          // 0: z walls use the segment kernel too (tuning builds)
+#endif
+constexpr int ZW_NQ = 6 + PU_MAX_TRACERS;     // carried face values: rho, w, p, then the advected fields
+
+template <int ORD, class T>
+__global__ __launch_bounds__(ZW_BX * ZW_BS) void k_sweep_zwall(const T *__restrict__ prim,
+                                                             const uint8_t *__restrict__ immf,
+                                                             const uint8_t *__restrict__ any10, T *__restrict__ tend,
+                                                             SweepParams P) {
+  constexpr int BX = ZW_BX, BS = ZW_BS, NT = BX * BS;
+  __shared__ T bufA0[NT], bufA1[NT], bufA2[NT], bufB0[NT], bufB1[NT];
+  __shared__ T carQ[2][ZW_NQ][BX], carF[2][ZW_NQ][BX];     // by chunk parity
+
+  const int tx = threadIdx.x, ty = threadIdx.y, lin = ty * BX + tx;
+  unsigned bx, by, bz;
+  swizzled_block(bx, by, bz);
+  const int i = bx * BX + tx, j = by, nz = P.nz;
+  const bool col_ok = i < P.nx;
+  const long long sstride = P.nxh * P.nyh;
+  const T rd = (T)P.rd;
+  const int nchunk = (nz + BS - 1) / BS;
+  const int nB = 3 + P.ntr;
+
+  for (int q = 0; q < nchunk; q++) {
+    const int r = q * BS + ty;
+    const bool valid = col_ok && r < nz;
+    const bool bot = r == 0, top = valid && r == nz - 1;
+    const bool carry_in = ty == 0 && q > 0;          // face below r: values of the previous chunk
+    const bool carry_out = ty == BS - 1;
+    const int pin = (q + 1) & 1, pout = q & 1;
+    const long long base = ((long long)(r + P.hp) * P.nyh + (j + P.hp)) * P.nxh + (i + P.ox);
+    const long long cid = ((long long)r * P.ny + j) * P.nx + i;
+
+    StencilFlags<ORD> sf;
+    if (valid) sf.setup(immf, base, sstride, any10[cid] != 0, P.weno_all);
+    auto recon = [&](int l, bool modify, T &qL, T &qR) {
+      sf.template recon<T>(prim + (long long)l * P.nh + base, sstride, modify, qL, qR);
+    };
+    T hr = 0, ht = 0;
+    if (valid) { hr = (T)P.hy_re[r]; ht = (T)P.hy_te[r]; }
+
+    // ---- phase A: density, w, pressure -> acoustic upwinding ----
+    T qLr = 0, qRr = 0, qLu = 0, qRu = 0, qLp = 0, qRp = 0;
+    if (valid) {
+      recon(idR, false, qLr, qRr);
+      recon(idW, false, qLu, qRu);
+      recon(idP, true, qLp, qRp);
+    }
+    bufA0[lin] = qRr; bufA1[lin] = qRu; bufA2[lin] = qRp;
+    __syncthreads();
+    Upw<T> up{T(0), T(0), true};
+    T Fr = 0, Fu = 0;
+    if (valid) {
+      T rL, uL, pL;
+      if (carry_in) { rL = carQ[pin][0][tx]; uL = carQ[pin][1][tx]; pL = carQ[pin][2][tx]; }
+      else if (ty > 0) { rL = bufA0[lin - BX]; uL = bufA1[lin - BX]; pL = bufA2[lin - BX]; }
+      else { rL = 0; uL = 0; pL = 0; }
+      T rR = qLr, uR = qLu, pR = qLp;
+      if (bot) { rL = rR; uL = 0; uR = 0; pL = pR; }
+      up = riemann(rL, uL, pL, rR, uR, pR, hr);
+      Fr = up.ru;
+      Fu = up.ru * (up.left ? uL : uR) + up.p;
+    }
+    bufB0[lin] = Fr; bufB1[lin] = Fu;
+    __syncthreads();
+    if (valid) {
+      if (r > 0) {
+        const T Fr0 = ty > 0 ? bufB0[lin - BX] : carF[pin][0][tx];
+        const T Fu0 = ty > 0 ? bufB1[lin - BX] : carF[pin][1][tx];
+        tend[(long long)idR * P.n3 + cid - (long long)P.nx * P.ny] = -(Fr - Fr0) * rd;
+        tend[(long long)idW * P.n3 + cid - (long long)P.nx * P.ny] = -(Fu - Fu0) * rd;
+      }
+      if (top) {
+        tend[(long long)idR * P.n3 + cid] = -(T(0) - Fr) * rd;
+        tend[(long long)idW * P.n3 + cid] = -(qRp - Fu) * rd;
+      }
+    }
+    if (carry_out) {
+      carQ[pout][0][tx] = qRr; carQ[pout][1][tx] = qRu; carQ[pout][2][tx] = qRp;
+      carF[pout][0][tx] = Fr; carF[pout][1][tx] = Fu;
+    }
+
+    // ---- phase B: the advected fields, one at a time ----
+    for (int b = 0; b < nB; b++) {
+      int l, fidx;
+      bool modify = false;
+      if (b == 0) { l = idU; fidx = idU; modify = true; }
+      else if (b == 1) { l = idV; fidx = idV; modify = true; }
+      else if (b == 2) { l = idT; fidx = idT; }
+      else { l = NUM_STATE + 1 + (b - 3); fidx = NUM_STATE + (b - 3); }
+      T qL = 0, qR = 0;
+      if (valid) recon(l, modify, qL, qR);
+      bufA0[lin] = qR;
+      __syncthreads();
+      T F = 0;
+      if (valid) {
+        T vL = carry_in ? carQ[pin][3 + b][tx] : (ty > 0 ? bufA0[lin - BX] : T(0)), vR = qL;
+        if (bot) vL = vR;
+        if (bot && l == idT) { vL = 0; vR = 0; }
+        if (l == idT) { vL += ht; vR += ht; }
+        F = up.ru * (up.left ? vL : vR);
+      }
+      bufB0[lin] = F;
+      __syncthreads();
+      if (valid) {
+        T *tp = tend + (long long)fidx * P.n3 + cid;
+        if (r > 0) tp[-(long long)P.nx * P.ny] = -(F - (ty > 0 ? bufB0[lin - BX] : carF[pin][2 + b][tx])) * rd;
+        if (top) *tp = -(T(0) - F) * rd;
+      }
+      if (carry_out) { carQ[pout][3 + b][tx] = qR; carF[pout][2 + b][tx] = F; }
+    }
   }
 }
 
@@ -729,7 +875,10 @@ static int sweep_one(pu_ctx *ctx, hipStream_t s, int mode, int stage, double dt,
   FuseArgs<T> FA{};
   const T *prim = (const T *)ctx->d_prim;
   T *tend = (T *)ctx->d_tend;
-  if (mode == 2) {
+  if (PU_ZWALL && DIR == 2 && mode == 0 && ctx->cfg.bc_z == PU_BC_SOLID_WALL) {
+    dim3 zb(ZW_BX, ZW_BS, 1), zg((unsigned)((g.nx + ZW_BX - 1) / ZW_BX), (unsigned)g.ny, 1);
+    hipLaunchKernelGGL((k_sweep_zwall<ORD, T>), zg, zb, 0, s, prim, ctx->d_imm_flag, ctx->d_any10, tend, P);
+  } else if (mode == 2) {
     FA.U = (T *)ctx->d_U; FA.Ulo = (T *)ctx->d_Ulo; FA.Us = (T *)ctx->d_Us; FA.Uslo = (T *)ctx->d_Uslo;
     FA.C = combine_params(ctx);
     FA.C.stage = stage; FA.C.dt = dt; FA.C.nrelax = nrelax;
