@@ -1,6 +1,7 @@
 // capi.hip -- the extern "C" entry points of libporturb (declared in include/porturb.h).
 #include "pu_common.h"
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <cstdio>
 #include <algorithm>
@@ -91,6 +92,10 @@ int pu_create(const pu_config *cfg, void *stream, pu_ctx **out) {
   pu_ctx *c = new pu_ctx();
   c->cfg = *cfg;
   c->stream = (hipStream_t)stream;
+  if (getenv("PORTURB_SEGMENT_SWEEPS")) {
+    const char *dirs = getenv("PORTURB_SEGMENT_DIRS");      // optional subset, e.g. "z" or "xy"
+    for (int d = 0; d < 3; d++) c->segment_sweeps[d] = !dirs || strchr(dirs, "xyz"[d]) != nullptr;
+  }
   Grid &g = c->g;
   g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
   g.hs = (cfg->ord - 1) / 2;

@@ -51,6 +51,9 @@ struct pu_ctx {
   int nf = 0;                      // nv + 1           (reconstructed fields incl. pressure)
   bool dycore_ready = false;
   bool les_ready = false;
+  // sweeps through the segment kernel (environment variable PORTURB_SEGMENT_SWEEPS set when the
+  // context is created): the whole-line kernels are checked against it
+  bool segment_sweeps[3] = {false, false, false};   // x, y, z (PORTURB_SEGMENT_DIRS narrows it)
   // hydrostatic background, FP64 (host + device copies)
   std::vector<double> hy_cells[3];     // dens, theta, pressure: [nz+2hs]
   std::vector<double> hy_edges[3];     // [nz+1]

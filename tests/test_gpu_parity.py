@@ -48,7 +48,19 @@ CASES = [
     dict(ord=5, nx=12, ny=10, nz=51),
     dict(ord=9, nx=16, ny=10, nz=101),
     dict(ord=7, nx=12, ny=10, nz=128),
+    # lines longer than one chunk of the whole-line x/y sweeps (64 cells in x, 32 in y), ragged ends,
+    # a last chunk holding only the periodic end cell, and a two-cell periodic line
+    dict(ord=9, nx=150, ny=70, nz=12),
+    dict(ord=5, nx=129, ny=65, nz=10, bc_z=pa.PU_BC_PERIODIC),
+    dict(ord=3, nx=22, ny=2, nz=10),
 ]
+
+
+# The 150x70x12 case's w tendency differs from the oracle by 2.5e-11 of its largest value at its
+# worst cell (identically for the segment and the whole-line sweeps).  The GPU evaluates the
+# smoothness indicators from first differences, the oracle (as the reference) from polynomial
+# coefficients of the raw values, whose rounding grows with a field's magnitude over its variation.
+TOL = {str(dict(ord=9, nx=150, ny=70, nz=12)): 5e-11}
 
 
 @pytest.mark.parametrize("case", CASES, ids=[str(c) for c in CASES])
@@ -56,13 +68,13 @@ def test_tendencies_match_oracle(case):
     kw = dict(case)
     nz, nx, ny = kw.pop("nz", 16), kw.pop("nx", 22), kw.pop("ny", 20)
     cfg = small_config(nx=nx, ny=ny, nz=nz, tracers=[WV, TKE, P1], **kw)
-    st, imm = random_state(cfg, seed=21, buildings=3)
+    st, imm = random_state(cfg, seed=21, buildings=3 if min(nx, ny) > 4 else 0)
     print("case", case, flush=True)
     d, cs = gpu_setup(cfg, st, imm)
     g = d.tendencies(cs)
     o = oracle_py.Dycore(cfg, st, imm).tendencies(st)
     err = np.abs(g - o).reshape(g.shape[0], -1).max(axis=1) / tend_scale(o)
-    assert err.max() < 1e-11, err
+    assert err.max() < TOL.get(str(case), 1e-11), err
 
 
 @pytest.mark.parametrize("ord_", (5, 9))
@@ -78,9 +90,10 @@ def test_hydrostatics_and_immersed_masks_match(ord_):
     assert (ga == oa).all() and (gt == ot).all()
 
 
-@pytest.mark.parametrize("ord_,steps,les,nz", [(5, 10, False, 18), (9, 4, True, 18), (9, 3, True, 70)])
-def test_time_steps_match_oracle(ord_, steps, les, nz):
-    cfg = small_config(nx=24, ny=20, nz=nz, ord=ord_, tracers=[WV, TKE, P1, P2])
+@pytest.mark.parametrize("ord_,steps,les,nz,nx,ny", [(5, 10, False, 18, 24, 20), (9, 4, True, 18, 24, 20),
+                                                     (9, 3, True, 70, 24, 20), (9, 2, True, 12, 130, 40)])
+def test_time_steps_match_oracle(ord_, steps, les, nz, nx, ny):
+    cfg = small_config(nx=nx, ny=ny, nz=nz, ord=ord_, tracers=[WV, TKE, P1, P2])
     st, imm = random_state(cfg, seed=33, buildings=3)
     d, cs = gpu_setup(cfg, st, imm)
     od = oracle_py.Dycore(cfg, st, imm)
