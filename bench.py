@@ -84,6 +84,20 @@ def cpu_baseline(cfg_full, les, seconds_budget=25.0):
                       (cfg["name"].split("_1024")[0], cfg["nx"], cfg["ny"], cfg["nz"], steps, t_tot)}
 
 
+def sweep_kernel_name(cfg, fam):
+    """The kernel that runs sweep family `fam` (the launch choice of dycore_kernels.hip sweep_one):
+    z between solid walls -> k_sweep_zwall, periodic y lines in FP64 -> k_sweep_pline, otherwise the
+    segment kernel k_sweep<ORD, T, DIR, MODE> (x finishes the stage: MODE 2)."""
+    import porturb_amd as pa
+    t = "double" if cfg["precision"] == pa.PU_FP64 else "float"
+    if fam == "weno_z" and cfg["bc_z"] == pa.PU_BC_SOLID_WALL:
+        return "k_sweep_zwall<%d, %s>" % (cfg["ord"], t)
+    if fam == "weno_y" and t == "double" and cfg["bc_y"] == pa.PU_BC_PERIODIC and cfg["ny"] >= 2:
+        return "k_sweep_pline<%d, %s, 1, 1>" % (cfg["ord"], t)
+    d = {"weno_x": 0, "weno_y": 1, "weno_z": 2}[fam]
+    return "k_sweep<%d, %s, %d, %d>" % (cfg["ord"], t, d, {0: 2, 1: 1, 2: 0}[d])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,13 +171,7 @@ def main():
         fam = json.load(open(tfile)).get("families", {}).get(dom)
         if fam:
             traffic = fam["read"] + fam["write"]
-    tname = "double" if fp64 else "float"
-    if dom == "weno_z" and cfg["bc_z"] == pa.PU_BC_SOLID_WALL:
-        kname = "k_sweep_zwall<%d, %s>" % (cfg["ord"], tname)          # whole-column z sweep
-    else:
-        kname = "k_sweep<%d, %s, %d, %d>" % (cfg["ord"], tname, {"weno_x": 0, "weno_y": 1, "weno_z": 2}[dom],
-                                             {"weno_x": 2, "weno_y": 1, "weno_z": 0}[dom])
-    roof = {"bound": "valu", "kernel": kname,
+    roof = {"bound": "valu", "kernel": sweep_kernel_name(cfg, dom),
             "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic,
             "flops_per_launch": flops_launch, "avg_launch_ms": round(avg_ms, 4),
