@@ -69,26 +69,26 @@ __host__ __device__ constexpr int unit_of_k(int k) {
 // GELU(x) = x Phi(x), Phi(x) = (1 + erf(x / sqrt 2)) / 2, with erf from Abramowitz & Stegun,
 // Handbook of Mathematical Functions, formula 7.1.26 (|error| <= 1.5e-7, far below the BF16
 // rounding of the activations): for z >= 0, erf(z) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5))))
-// exp(-z^2), t = 1 / (1 + p z).  Branch-free: e = 1 - erf(|x| / sqrt 2) is formed once and
-// Phi = 1 - e/2 for x >= 0, e/2 for x < 0.
+// exp(-z^2), t = 1 / (1 + p z).  phi_tail returns E = (1 - erf(|x| / sqrt 2)) / 2 (the factors
+// 1/sqrt 2 and 1/2 folded into p and the a_i), so Phi = 1 - E for x >= 0 and E for x < 0, and
+// GELU(x) = max(x, 0) - |x E| without a branch or select.
 __device__ __forceinline__ float phi_tail(float x, float &ex) {
-  const float z = fabsf(x) * 0.70710678118654752f;
-  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.f));
-  const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
-                             0.254829592f);
-  ex = __builtin_amdgcn_exp2f(-z * z * 1.44269504088896341f);           // exp(-x^2 / 2)
-  return poly * ex;                                                     // 1 - erf(|x| / sqrt 2)
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, fabsf(x), 1.f));
+  const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 0.5f * 1.061405429f, 0.5f * -1.453152027f), 0.5f * 1.421413741f),
+                                      0.5f * -0.284496736f),
+                             0.5f * 0.254829592f);
+  ex = __builtin_amdgcn_exp2f(x * x * (-0.5f * 1.44269504088896341f));  // exp(-x^2 / 2)
+  return poly * ex;
 }
 __device__ __forceinline__ float gelu(float x) {
   float ex;
   const float e = phi_tail(x, ex);
-  const float phi = x >= 0.f ? fmaf(-0.5f, e, 1.f) : 0.5f * e;
-  return x * phi;
+  return fmaxf(x, 0.f) - fabsf(x * e);
 }
 __device__ __forceinline__ float gelu_deriv(float x) {                   // Phi(x) + x phi(x)
   float ex;
   const float e = phi_tail(x, ex);
-  const float phi = x >= 0.f ? fmaf(-0.5f, e, 1.f) : 0.5f * e;
+  const float phi = x >= 0.f ? 1.f - e : e;
   return fmaf(x * 0.39894228040143268f, ex, phi);
 }
 
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   __shared__ __align__(16) short sW1[NH * NINP];   // row stride 112: block 3 reads 16 columns
   __shared__ __align__(16) short sW2[NH * NH];  // columns in unit_of_k order
   __shared__ __align__(16) short sW3[NOUTP * NH];
-  __shared__ float sB[NH + NH + NOUTP];
+  __shared__ __align__(16) float sB[NH + NH + NOUTP];
   __shared__ float sFm[K1];
   __shared__ float sY[4][16][NOUTP + 1];
   __shared__ T sQ[4 * GBN];                     // u, v, w, theta of the group's neighbourhood
@@ -551,7 +551,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     bfx4 a1[4], a2[4];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
-      fx4 acc = {0.f, 0.f, 0.f, 0.f};
+      fx4 acc = *reinterpret_cast<const fx4 *>(sB + 16 * mt + 4 * g);   // the bias starts the sum
 #pragma unroll
       for (int kb = 0; kb < 4; kb++) {
         // inputs 112..127 do not exist (their B values are 0): lanes 32..63 of block 3 use a zero A
@@ -559,22 +559,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         acc = mfma32(wa, xb[kb], acc);
       }
 #pragma unroll
-      for (int r = 0; r < 4; r++) a1[mt][r] = bf16_bits(gelu(acc[r] + sB[16 * mt + 4 * g + r]));
+      for (int r = 0; r < 4; r++) a1[mt][r] = bf16_bits(gelu(acc[r]));
     }
     // layer 2: the B fragment of K block kb2 is (a1[2 kb2], a1[2 kb2 + 1]) in unit_of_k order
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
-      fx4 acc = {0.f, 0.f, 0.f, 0.f};
+      fx4 acc = *reinterpret_cast<const fx4 *>(sB + NH + 16 * mt + 4 * g);
 #pragma unroll
       for (int kb = 0; kb < 2; kb++)
         acc = mfma32(ld8(sW2 + (16 * mt + r16) * NH + 32 * kb + 8 * g), cat8(a1[2 * kb], a1[2 * kb + 1]), acc);
 #pragma unroll
-      for (int r = 0; r < 4; r++) a2[mt][r] = bf16_bits(gelu(acc[r] + sB[NH + 16 * mt + 4 * g + r]));
+      for (int r = 0; r < 4; r++) a2[mt][r] = bf16_bits(gelu(acc[r]));
     }
-    fx4 y = {0.f, 0.f, 0.f, 0.f};
+    fx4 y = *reinterpret_cast<const fx4 *>(sB + 2 * NH + 4 * g);
 #pragma unroll
     for (int kb = 0; kb < 2; kb++) y = mfma32(ld8(sW3 + r16 * NH + 32 * kb + 8 * g), cat8(a2[2 * kb], a2[2 * kb + 1]), y);
-    for (int r = 0; r < 4; r++) sY[w][r16][4 * g + r] = y[r] + sB[2 * NH + 4 * g + r];
+    for (int r = 0; r < 4; r++) sY[w][r16][4 * g + r] = y[r];
     __syncthreads();
     if (g == 0 && in) {
       const long long id = ((long long)k * A.ny + j) * A.nx + i;
