@@ -30,7 +30,8 @@ struct LesParams {
   const unsigned char *sflags;
 };
 
-__device__ __forceinline__ int wrapl(int i, int n) { i %= n; return i < 0 ? i + n : i; }
+// periodic image of an index at most one period outside [0, n) (every use here is one cell away)
+__device__ __forceinline__ int wrapl(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // fields of the closure box: 0 rho_d, 1 u, 2 v, 3 w, 4 theta, 5 K, 6+q diffused tracer mixing ratios
 // (values computed in FP64, stored as T: double, or float for the FP32 dycore mode)
@@ -267,9 +268,10 @@ __global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, dou
       const int q = threadIdx.x + 256 * e;
       int i = i0 + q % LBX - 1, j = j0 + (q / LBX) % LBY - 1, k = k0 + q / (LBX * LBY) - 1;
       bool ok = q < NB;
-      if (i < 0 || i >= P.nx) { if (P.bc_x == PU_BC_PERIODIC) i = wrapl(i, P.nx); else ok = false; }
-      if (j < 0 || j >= P.ny) { if (P.bc_y == PU_BC_PERIODIC) j = wrapl(j, P.ny); else ok = false; }
-      if (k < 0 || k >= P.nz) { if (P.bc_z == PU_BC_PERIODIC) k = wrapl(k, P.nz); else ok = false; }
+      // points more than one cell past the domain belong to no cell's faces (ragged last tiles)
+      if (i < 0 || i >= P.nx) { if (P.bc_x == PU_BC_PERIODIC && i <= P.nx) i = wrapl(i, P.nx); else ok = false; }
+      if (j < 0 || j >= P.ny) { if (P.bc_y == PU_BC_PERIODIC && j <= P.ny) j = wrapl(j, P.ny); else ok = false; }
+      if (k < 0 || k >= P.nz) { if (P.bc_z == PU_BC_PERIODIC && k <= P.nz) k = wrapl(k, P.nz); else ok = false; }
       fb[e] = ok ? P.sflags[((long long)k * P.ny + j) * P.nx + i] : (unsigned char)0;
     }
 #pragma unroll
@@ -289,19 +291,30 @@ __global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, dou
   const int c = ((lk + 1) * LBY + (lj + 1)) * LBX + (li + 1);
   auto S = [&](int f, int idx) { return B.S(f, idx); };
   FaceCommon<T> fx0, fx1, fy0, fy1, fz0, fz1;
-  // global ids of the six neighbours (wrapped where periodic; out-of-domain ones carry flag 0)
-  auto gid = [&](int ii, int jj, int kk) {
-    ii = P.bc_x == PU_BC_PERIODIC ? wrapl(ii, P.nx) : ii;
-    jj = P.bc_y == PU_BC_PERIODIC ? wrapl(jj, P.ny) : jj;
-    kk = P.bc_z == PU_BC_PERIODIC ? wrapl(kk, P.nz) : kk;
-    return ((long long)kk * P.ny + jj) * P.nx + ii;
-  };
-  face_flux<0>(B, P, c, kh, jh, ih, gid(i - 1, j, k), id, fx0);
-  face_flux<0>(B, P, c + sx, kh, jh, ih + 1, id, gid(i + 1, j, k), fx1);
-  face_flux<1>(B, P, c, kh, jh, ih, gid(i, j - 1, k), id, fy0);
-  face_flux<1>(B, P, c + sy, kh, jh + 1, ih, id, gid(i, j + 1, k), fy1);
-  face_flux<2>(B, P, c, kh, jh, ih, gid(i, j, k - 1), id, fz0);
-  face_flux<2>(B, P, c + sz, kh + 1, jh, ih, id, gid(i, j, k + 1), fz1);
+  // global ids of the six neighbours, read only by the learned closure (wrapped where periodic;
+  // out-of-domain ones carry flag 0)
+  long long gn[6] = {0, 0, 0, 0, 0, 0};
+  if (P.sflags) {
+    const int im = P.bc_x == PU_BC_PERIODIC ? wrapl(i - 1, P.nx) : i - 1;
+    const int ip = P.bc_x == PU_BC_PERIODIC ? wrapl(i + 1, P.nx) : i + 1;
+    const int jm = P.bc_y == PU_BC_PERIODIC ? wrapl(j - 1, P.ny) : j - 1;
+    const int jp = P.bc_y == PU_BC_PERIODIC ? wrapl(j + 1, P.ny) : j + 1;
+    const int km = P.bc_z == PU_BC_PERIODIC ? wrapl(k - 1, P.nz) : k - 1;
+    const int kp = P.bc_z == PU_BC_PERIODIC ? wrapl(k + 1, P.nz) : k + 1;
+    const long long row = (long long)P.nx, pln = (long long)P.nx * P.ny;
+    gn[0] = id + (im - i);
+    gn[1] = id + (ip - i);
+    gn[2] = id + (jm - j) * row;
+    gn[3] = id + (jp - j) * row;
+    gn[4] = id + (km - k) * pln;
+    gn[5] = id + (kp - k) * pln;
+  }
+  face_flux<0>(B, P, c, kh, jh, ih, gn[0], id, fx0);
+  face_flux<0>(B, P, c + sx, kh, jh, ih + 1, id, gn[1], fx1);
+  face_flux<1>(B, P, c, kh, jh, ih, gn[2], id, fy0);
+  face_flux<1>(B, P, c + sy, kh, jh + 1, ih, id, gn[3], fy1);
+  face_flux<2>(B, P, c, kh, jh, ih, gn[4], id, fz0);
+  face_flux<2>(B, P, c + sz, kh + 1, jh, ih, id, gn[5], fz1);
   const T h = T(0.5);
   const T rdx = (T)P.rdx, rdy = (T)P.rdy, rdz = (T)P.rdz;
   const T r2dx = h * rdx, r2dy = h * rdy, r2dz = h * rdz;
