@@ -40,9 +40,12 @@ __global__ __launch_bounds__(256) void k_les_prim(const double *__restrict__ rd_
                                                   const double *__restrict__ vv, const double *__restrict__ ww,
                                                   const double *__restrict__ tt, TracerPtrsC trc,
                                                   T *__restrict__ F, LesParams P) {
-  long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= P.nh) return;
-  int ih = (int)(id % P.nxh), jh = (int)((id / P.nxh) % P.nyh), kh = (int)(id / (P.nxh * P.nyh));
+  // grid: (blocks over one halo'd level's nxh*nyh points, nzh levels)
+  const int nxyh = (int)(P.nxh * P.nyh);
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, kh = blockIdx.y;
+  if (p >= nxyh) return;
+  const int jh = p / (int)P.nxh, ih = p - jh * (int)P.nxh;
+  const long long id = (long long)kh * nxyh + p;
   int i = ih - 1, j = jh - 1, k = kh - 1;
   bool ox = i < 0 || i >= P.nx, oy = j < 0 || j >= P.ny;
   // x/y source (periodic exchange; precursor replaces only the face halos, corners stay periodic)
@@ -427,7 +430,8 @@ static int les_params(pu_ctx *ctx, const pu_coupler_state *cs, double dt, LesPar
 
 template <class T>
 static int launch_les_prim(pu_ctx *ctx, const pu_coupler_state *cs, const LesParams &P, const TracerPtrsC &tpc) {
-  hipLaunchKernelGGL(k_les_prim<T>, dim3((unsigned)((P.nh + 255) / 256)), dim3(256), 0, ctx->stream, cs->density_dry,
+  hipLaunchKernelGGL(k_les_prim<T>, dim3((unsigned)((P.nxh * P.nyh + 255) / 256), (unsigned)P.nzh), dim3(256), 0,
+                     ctx->stream, cs->density_dry,
                      cs->uvel, cs->vvel, cs->wvel, cs->temp, tpc, (T *)ctx->d_les, P);
   PU_HIP_CHECK(hipGetLastError());
   return PU_OK;

@@ -38,11 +38,13 @@ struct SfcParams {
   const double *prop, *rough, *temp;    // halo'd boxes
 };
 
+// grid: (blocks over one level's nx*ny cells, nz levels): no 64-bit division for the indices
 __global__ __launch_bounds__(256) void k_surface_fluxes(double *uu, double *vv, double *ww, double *tt, SfcParams P) {
-  const long long n3 = (long long)P.nx * P.ny * P.nz;
-  const long long id = (long long)xcd_swizzle(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-  if (id >= n3) return;
-  const int i = (int)(id % P.nx), j = (int)((id / P.nx) % P.ny), k = (int)(id / ((long long)P.nx * P.ny));
+  const int nxy = P.nx * P.ny;
+  const int p = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y;
+  if (p >= nxy) return;
+  const int j = p / P.nx, i = p - j * P.nx;
+  const long long id = (long long)k * nxy + p;
   const long long nxh = P.nx + 2, nyh = P.ny + 2;
   const long long c = ((long long)(k + 1) * nyh + (j + 1)) * nxh + (i + 1);
   const double u = uu[id], v = vv[id], w = ww[id], T = tt[id];
@@ -123,11 +125,10 @@ int pu_surface_fluxes_apply(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys) {
   P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
   P.dx = c.xlen / c.nx; P.dy = c.ylen / c.ny; P.dz = c.zlen / c.nz; P.dt = dt_phys;
   P.prop = ctx->d_imm_les_val; P.rough = ctx->d_sfc_rough; P.temp = ctx->d_sfc_temp;
-  const long long n3 = (long long)c.nx * c.ny * c.nz;
   {
     LaunchTimer lt(ctx, "surface");
-    hipLaunchKernelGGL(k_surface_fluxes, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, ctx->stream, cs->uvel, cs->vvel,
-                       cs->wvel, cs->temp, P);
+    hipLaunchKernelGGL(k_surface_fluxes, dim3((unsigned)((c.nx * c.ny + 255) / 256), (unsigned)c.nz), dim3(256), 0,
+                       ctx->stream, cs->uvel, cs->vvel, cs->wvel, cs->temp, P);
     PU_HIP_CHECK(hipGetLastError());
   }
   if (ctx->kernel_timing) collect_timers(ctx);
