@@ -180,13 +180,15 @@ int pu_dycore_time_step(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys) {
   double dt_dyn = pu_dycore_compute_time_step(ctx);
   int ncycles = (int)std::ceil(dt_phys / dt_dyn);
   dt_dyn = dt_phys / ncycles;
-  // convert + the relaxation that opens the first SSPRK3 step
-  if ((rc = launch_convert_c2d(ctx, cs, 1, s))) return rc;
   for (int cyc = 0; cyc < ncycles; cyc++) {
     for (int stage = 0; stage < 3; stage++) {
       const void *U = stage == 0 ? ctx->d_U : ctx->d_Us;
       const void *Ulo = stage == 0 ? ctx->d_Ulo : ctx->d_Uslo;
-      if ((rc = launch_prep(ctx, U, Ulo, s))) return rc;
+      // the first stage converts the coupler state (with the relaxation that opens the first
+      // SSPRK3 step) inside its prep
+      if (cyc == 0 && stage == 0) rc = launch_c2d_prep(ctx, cs, s);
+      else rc = launch_prep(ctx, U, Ulo, s);
+      if (rc) return rc;
       // the final relaxation of a step is followed by the opening relaxation of the next one
       int nrelax = (stage == 2 && cyc + 1 < ncycles) ? 2 : 1;
       // z, y sweeps accumulate tendencies; the x sweep finishes them and combines the stage

@@ -553,17 +553,15 @@ struct ConvParams {
   const double *hy_r, *hy_t, *relax_mult;
 };
 
-template <class ST>
-__global__ __launch_bounds__(256) void k_c2d(const double *__restrict__ rd_, const double *__restrict__ uu,
-                                             const double *__restrict__ vv, const double *__restrict__ ww,
-                                             const double *__restrict__ tt, TracerPtrs trc, ST *U, ST *Ulo,
-                                             ConvParams P) {
-  long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= P.n3) return;
-  int k = (int)(id / ((long long)P.nx * P.ny));
+// One cell of the coupler -> dynamics conversion (dynamics_rk.h:1251-1304): the conserved state
+// (rho, rho u, rho v, rho w, rho theta, rho q) of coupler cell id at level k into x[], with the
+// opening immersed relaxation when P.relax.  All loads come before any store of the caller (the
+// stores may alias the tracer arrays).
+__device__ __forceinline__ void c2d_cell(const double *__restrict__ rd_, const double *__restrict__ uu,
+                                         const double *__restrict__ vv, const double *__restrict__ ww,
+                                         const double *__restrict__ tt, const TracerPtrs &trc, long long id, int k,
+                                         const ConvParams &P, double *x) {
   double rho_d = rd_[id], u = uu[id], v = vv[id], w = ww[id], temp = tt[id];
-  // all tracer loads before the first store (the stores may alias them, so a load-store loop would
-  // wait on every load)
   double m[PU_MAX_TRACERS];
 #pragma unroll
   for (int tr = 0; tr < PU_MAX_TRACERS; tr++) m[tr] = tr < P.ntr ? trc.p[tr][id] : 0.;
@@ -576,24 +574,103 @@ __global__ __launch_bounds__(256) void k_c2d(const double *__restrict__ rd_, con
   double vals[NUM_STATE] = {rho, rho * u, rho * v, rho * w, rho * theta};
   double mult = P.relax ? P.relax_mult[id] : 0.;
   double tr_r = P.hy_r[P.hs + k], tr_rt = P.hy_r[P.hs + k] * P.hy_t[P.hs + k];
+#pragma unroll
   for (int l = 0; l < NUM_STATE; l++) {
-    double x = vals[l];
+    double xv = vals[l];
     if (P.relax) {
       double target = l == idR ? tr_r : (l == idT ? tr_rt : 0.);
-      x = x + (target - x) * mult;
+      xv = xv + (target - xv) * mult;
     }
-    store_cons(U, Ulo, l, id, P.n3, x);
+    x[l] = xv;
   }
 #pragma unroll
   for (int tr = 0; tr < PU_MAX_TRACERS; tr++) {
-    if (tr >= P.ntr) break;
-    double x = m[tr];
-    if (P.relax) {
-      x = x + (0. - x) * mult;
-      if ((P.positive_mask >> tr) & 1u) x = fmax(0., x);
+    double xv = m[tr];
+    if (tr < P.ntr && P.relax) {
+      xv = xv + (0. - xv) * mult;
+      if ((P.positive_mask >> tr) & 1u) xv = fmax(0., xv);
     }
-    store_cons(U, Ulo, NUM_STATE + tr, id, P.n3, x);
+    x[NUM_STATE + tr] = xv;
   }
+}
+
+template <class ST>
+__global__ __launch_bounds__(256) void k_c2d(const double *__restrict__ rd_, const double *__restrict__ uu,
+                                             const double *__restrict__ vv, const double *__restrict__ ww,
+                                             const double *__restrict__ tt, TracerPtrs trc, ST *U, ST *Ulo,
+                                             ConvParams P) {
+  // grid: (blocks over one level's nx*ny cells, nz levels)
+  const int nxy = P.nx * P.ny;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  if (p >= nxy) return;
+  const long long id = (long long)k * nxy + p;
+  double x[NUM_STATE + PU_MAX_TRACERS];
+  c2d_cell(rd_, uu, vv, ww, tt, trc, id, k, P, x);
+#pragma unroll
+  for (int l = 0; l < NUM_STATE + PU_MAX_TRACERS; l++)
+    if (l < NUM_STATE + P.ntr) store_cons(U, Ulo, l, id, P.n3, x[l]);
+}
+
+// The value of conserved field l as load_cons reads it back after store_cons (FP32 storage: the
+// float, plus the float compensation term for rho and rho theta).
+template <class ST> __device__ __forceinline__ double stored_cons(int l, double v) {
+  if constexpr (std::is_same<ST, float>::value) {
+    const float hi = (float)v;
+    if (l == idR || l == idT) return (double)hi + (double)(float)(v - (double)hi);
+    return (double)hi;
+  } else {
+    return v;
+  }
+}
+
+// The first stage's prep fused with the coupler -> dynamics conversion: every point of the
+// primitive-field box converts its source cell's coupler state (as k_c2d) and derives the primitive
+// fields from the conserved values as stored (as k_prep); interior points also store the
+// conserved state.  Saves reading the conserved state back (one pass over nv fields).
+template <class ST>
+__global__ __launch_bounds__(256) void k_c2d_prep(const double *__restrict__ rd_, const double *__restrict__ uu,
+                                                  const double *__restrict__ vv, const double *__restrict__ ww,
+                                                  const double *__restrict__ tt, TracerPtrs trc, ST *U, ST *Ulo,
+                                                  ST *__restrict__ prim, PrepParams P, ConvParams C) {
+  int ih = blockIdx.x * blockDim.x + threadIdx.x;
+  int jh = blockIdx.y * blockDim.y + threadIdx.y;
+  int kh = blockIdx.z;
+  if (ih >= P.nxh || jh >= P.nyh || kh >= P.nzh) return;
+  int i = ih - P.ox, j = jh - P.hp, k = kh - P.hp;
+  if (i < -P.hp || i >= P.nx + P.hp) return;          // row padding
+  bool ox = i < 0 || i >= P.nx, oy = j < 0 || j >= P.ny, oz = k < 0 || k >= P.nz;
+  if ((int)ox + (int)oy + (int)oz >= 2) return;
+  int is = i, js = j, ks = k;
+  bool zwall = false;
+  if (ox) is = P.bc_x == PU_BC_PERIODIC ? wrapi(i, P.nx) : clampi(i, 0, P.nx - 1);
+  if (oy) js = P.bc_y == PU_BC_PERIODIC ? wrapi(j, P.ny) : clampi(j, 0, P.ny - 1);
+  if (oz) {
+    if (P.bc_z == PU_BC_PERIODIC) ks = wrapi(k, P.nz);
+    else { ks = clampi(k, 0, P.nz - 1); zwall = true; }
+  }
+  long long sid = ((long long)ks * P.ny + js) * P.nx + is;
+  long long hid = ((long long)kh * P.nyh + jh) * P.nxh + ih;
+  double x[NUM_STATE + PU_MAX_TRACERS];
+  c2d_cell(rd_, uu, vv, ww, tt, trc, sid, ks, C, x);
+  if (!ox && !oy && !oz) {
+#pragma unroll
+    for (int l = 0; l < NUM_STATE + PU_MAX_TRACERS; l++)
+      if (l < NUM_STATE + P.ntr) store_cons(U, Ulo, l, sid, P.n3, x[l]);
+  }
+  double rho = stored_cons<ST>(idR, x[idR]);
+  double rt = stored_cons<ST>(idT, x[idT]);
+  int kc = P.hs + ks;
+  double r_r = 1. / rho;
+  prim[(long long)idP * P.nh + hid] = (ST)(P.C0 * pow(rt, P.gamma) - P.hy_p[kc]);
+  prim[(long long)idR * P.nh + hid] = (ST)(rho - P.hy_r[kc]);
+  prim[(long long)idU * P.nh + hid] = (ST)(stored_cons<ST>(idU, x[idU]) * r_r);
+  prim[(long long)idV * P.nh + hid] = (ST)(stored_cons<ST>(idV, x[idV]) * r_r);
+  prim[(long long)idW * P.nh + hid] = zwall ? (ST)0 : (ST)(stored_cons<ST>(idW, x[idW]) * r_r);
+  prim[(long long)idT * P.nh + hid] = zwall ? (ST)0 : (ST)(rt * r_r - P.hy_t[kc]);
+#pragma unroll
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++)
+    if (tr < P.ntr)
+      prim[(long long)(NUM_STATE + 1 + tr) * P.nh + hid] = (ST)(stored_cons<ST>(NUM_STATE + tr, x[NUM_STATE + tr]) * r_r);
 }
 
 template <class ST>
@@ -795,8 +872,7 @@ int launch_convert_c2d(pu_ctx *ctx, const pu_coupler_state *cs, int relax, hipSt
   P.relax = relax;
   TracerPtrs tp;
   for (int tr = 0; tr < PU_MAX_TRACERS; tr++) tp.p[tr] = tr < ctx->cfg.num_tracers ? cs->tracers[tr] : nullptr;
-  long long n = ctx->g.n3();
-  dim3 grd((unsigned)((n + 255) / 256)), blk(256);
+  dim3 grd((unsigned)((ctx->g.nx * ctx->g.ny + 255) / 256), (unsigned)ctx->g.nz), blk(256);
   if (ctx->cfg.precision == PU_FP32)
     hipLaunchKernelGGL(k_c2d<float>, grd, blk, 0, s, cs->density_dry, cs->uvel, cs->vvel, cs->wvel, cs->temp, tp,
                        (float *)ctx->d_U, (float *)ctx->d_Ulo, P);
@@ -805,6 +881,32 @@ int launch_convert_c2d(pu_ctx *ctx, const pu_coupler_state *cs, int relax, hipSt
                        (double *)ctx->d_U, (double *)ctx->d_Ulo, P);
   PU_HIP_CHECK(hipGetLastError());
   return PU_OK;
+}
+
+template <class ST>
+static int c2d_prep_t(pu_ctx *ctx, const pu_coupler_state *cs, hipStream_t s) {
+  const pu::Grid &g = ctx->g;
+  PrepParams P;
+  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ox = g.ox; P.ntr = ctx->cfg.num_tracers;
+  P.bc_x = ctx->cfg.bc_x; P.bc_y = ctx->cfg.bc_y; P.bc_z = ctx->cfg.bc_z;
+  P.n3 = g.n3(); P.nxh = g.nxh; P.nyh = g.nyh; P.nzh = g.nzh; P.nh = g.nh();
+  P.C0 = ctx->cfg.C0; P.gamma = ctx->cfg.gamma_d;
+  P.hy_r = hy_cells(ctx, 0); P.hy_t = hy_cells(ctx, 1); P.hy_p = hy_cells(ctx, 2);
+  ConvParams C = conv_params(ctx);
+  C.relax = 1;
+  TracerPtrs tp;
+  for (int tr = 0; tr < PU_MAX_TRACERS; tr++) tp.p[tr] = tr < ctx->cfg.num_tracers ? cs->tracers[tr] : nullptr;
+  dim3 blk(64, 4, 1), grd((unsigned)((g.nxh + 63) / 64), (unsigned)((g.nyh + 3) / 4), (unsigned)g.nzh);
+  LaunchTimer lt(ctx, "prep");
+  hipLaunchKernelGGL(k_c2d_prep<ST>, grd, blk, 0, s, cs->density_dry, cs->uvel, cs->vvel, cs->wvel, cs->temp, tp,
+                     (ST *)ctx->d_U, (ST *)ctx->d_Ulo, (ST *)ctx->d_prim, P, C);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
+// the conversion with the opening relaxation of a time step, fused with its first stage's prep
+int launch_c2d_prep(pu_ctx *ctx, const pu_coupler_state *cs, hipStream_t s) {
+  return ctx->cfg.precision == PU_FP32 ? c2d_prep_t<float>(ctx, cs, s) : c2d_prep_t<double>(ctx, cs, s);
 }
 
 int launch_convert_d2c(pu_ctx *ctx, pu_coupler_state *cs, hipStream_t s) {

@@ -165,6 +165,7 @@ void collect_timers(pu_ctx *ctx);
 
 // Launchers implemented in the kernel translation units.
 int launch_prep(pu_ctx *ctx, const void *U, const void *Ulo, hipStream_t s);
+int launch_c2d_prep(pu_ctx *ctx, const pu_coupler_state *cs, hipStream_t s);
 // Sweeps z, y, x.  stage < 0: tendencies into d_tend; stage 0..2: the x sweep also applies the
 // stage combination (sources, SSPRK3 stage, clip, nrelax immersed relaxations) for every cell it
 // finishes
