@@ -86,13 +86,13 @@ def cpu_baseline(cfg_full, les, seconds_budget=25.0):
 
 def sweep_kernel_name(cfg, fam):
     """The kernel that runs sweep family `fam` (the launch choice of dycore_kernels.hip sweep_one):
-    z between solid walls -> k_sweep_zwall, periodic y lines in FP64 -> k_sweep_pline, otherwise the
-    segment kernel k_sweep<ORD, T, DIR, MODE> (x finishes the stage: MODE 2)."""
+    z between solid walls -> k_sweep_zwall, periodic y lines -> k_sweep_pline, otherwise the segment
+    kernel k_sweep<ORD, T, DIR, MODE> (x finishes the stage: MODE 2)."""
     import porturb_amd as pa
     t = "double" if cfg["precision"] == pa.PU_FP64 else "float"
     if fam == "weno_z" and cfg["bc_z"] == pa.PU_BC_SOLID_WALL:
         return "k_sweep_zwall<%d, %s>" % (cfg["ord"], t)
-    if fam == "weno_y" and t == "double" and cfg["bc_y"] == pa.PU_BC_PERIODIC and cfg["ny"] >= 2:
+    if fam == "weno_y" and cfg["bc_y"] == pa.PU_BC_PERIODIC and cfg["ny"] >= 2:
         return "k_sweep_pline<%d, %s, 1, 1>" % (cfg["ord"], t)
     d = {"weno_x": 0, "weno_y": 1, "weno_z": 2}[fam]
     return "k_sweep<%d, %s, %d, %d>" % (cfg["ord"], t, d, {0: 2, 1: 1, 2: 0}[d])

@@ -24,38 +24,6 @@ struct PrepParams {
   const double *hy_r, *hy_t, *hy_p;     // cells [nz+2hs]
 };
 
-struct CombineParams {
-  int nx, ny, nz, hs, ntr, stage, nrelax, enable_gravity, coriolis;
-  long long n3;
-  double dt, grav, fcor;
-  const double *hy_r, *hy_t;        // cells
-  const double *relax;              // [n3]
-  unsigned int positive_mask;       // bit tr set if tracer tr is positive-definite
-};
-
-// Conserved value as double: FP32 storage adds the compensation term of rho (l=0), rho*theta (l=4).
-template <class ST>
-__device__ __forceinline__ double load_cons(const ST *U, const ST *Ulo, int l, long long id, long long n3) {
-  double v = (double)U[(long long)l * n3 + id];
-  if constexpr (std::is_same<ST, float>::value) {
-    if (l == idR) v += (double)Ulo[id];
-    else if (l == idT) v += (double)Ulo[n3 + id];
-  }
-  return v;
-}
-
-template <class ST>
-__device__ __forceinline__ void store_cons(ST *U, ST *Ulo, int l, long long id, long long n3, double v) {
-  if constexpr (std::is_same<ST, float>::value) {
-    float hi = (float)v;
-    U[(long long)l * n3 + id] = hi;
-    if (l == idR) Ulo[id] = (float)(v - (double)hi);
-    else if (l == idT) Ulo[n3 + id] = (float)(v - (double)hi);
-  } else {
-    U[(long long)l * n3 + id] = v;
-  }
-}
-
 // ------------------------------------------------------------------------------------------------
 // prep: primitive fields with halos (cross-shaped: the halo corners are never read)
 // ------------------------------------------------------------------------------------------------
@@ -93,60 +61,6 @@ __global__ __launch_bounds__(256) void k_prep(const ST *__restrict__ U, const ST
   for (int tr = 0; tr < P.ntr; tr++)
     prim[(long long)(NUM_STATE + 1 + tr) * P.nh + hid] =
         (ST)((double)U[(long long)(NUM_STATE + tr) * P.n3 + sid] * r_r);
-}
-
-// ------------------------------------------------------------------------------------------------
-// SSPRK3 stage combination of one conserved field of one cell (time_step_rk_3_3,
-// dynamics_rk.h:279-367): gravity and Coriolis sources, the stage's convex combination, the
-// positivity clip of the last stage and nrelax immersed relaxations (enforce_immersed_boundaries,
-// :370-473).  rho_in, u_in, v_in are the stage-input state of the cell, read before any field of the
-// cell is overwritten (stage 1 writes Us in place).
-// ------------------------------------------------------------------------------------------------
-template <class ST> struct FuseArgs {
-  ST *U, *Ulo, *Us, *Uslo;
-  CombineParams C;
-};
-
-// un, us: the field's values in U and (stages 1, 2) Us, read by the caller ahead of time
-template <class ST>
-__device__ __forceinline__ void combine_field_v(const FuseArgs<ST> &F, int l, long long id, double tn, double un,
-                                                double us, double rho_in, double u_in, double v_in, double mult,
-                                                double tr_r, double tr_rt) {
-  const CombineParams &P = F.C;
-  if (l == idW && P.enable_gravity) tn += -P.grav * (rho_in - tr_r);
-  if (P.coriolis && l == idU) tn += P.fcor * v_in;
-  if (P.coriolis && l == idV) tn -= P.fcor * u_in;
-  double v;
-  if (P.stage == 0) {
-    v = un + P.dt * tn;
-  } else if (P.stage == 1) {
-    v = (3. / 4.) * un + (1. / 4.) * us + (1. / 4.) * P.dt * tn;
-  } else {
-    v = (1. / 3.) * un + (2. / 3.) * us + (2. / 3.) * P.dt * tn;
-  }
-  bool positive = l >= NUM_STATE && ((P.positive_mask >> (l - NUM_STATE)) & 1u);
-  if (P.stage == 2 && positive) v = fmax(0., v);
-  double target = l == idR ? tr_r : (l == idT ? tr_rt : 0.);
-  for (int r = 0; r < P.nrelax; r++) {
-    v = v + (target - v) * mult;
-    if (positive) v = fmax(0., v);
-  }
-  if (P.stage == 2) store_cons(F.U, F.Ulo, l, id, P.n3, v);
-  else store_cons(F.Us, F.Uslo, l, id, P.n3, v);
-}
-
-template <class ST>
-__device__ __forceinline__ void stage_inputs(const FuseArgs<ST> &F, long long id, int k, double &rho_in, double &u_in,
-                                             double &v_in, double &mult, double &tr_r, double &tr_rt) {
-  const CombineParams &P = F.C;
-  const ST *Sin = P.stage == 0 ? F.U : F.Us;
-  const ST *Sinlo = P.stage == 0 ? F.Ulo : F.Uslo;
-  rho_in = load_cons(Sin, Sinlo, idR, id, P.n3);
-  u_in = P.coriolis ? load_cons(Sin, Sinlo, idU, id, P.n3) : 0.;
-  v_in = P.coriolis ? load_cons(Sin, Sinlo, idV, id, P.n3) : 0.;
-  mult = P.relax[id];
-  tr_r = P.hy_r[P.hs + k];
-  tr_rt = P.hy_r[P.hs + k] * P.hy_t[P.hs + k];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -329,216 +243,6 @@ __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep
     bufB0[lin] = F;
     __syncthreads();
     if (has_tend) accumulate(fidx, -(bufB0[lin + nbs] - bufB0[lin]) * rd, pf);
-  }
-}
-
-// ------------------------------------------------------------------------------------------------
-// x or y sweep along a periodic line: one block walks whole lines in chunks (x: 4 rows, chunks of
-// 64 cells; y: 8 columns, chunks of 32), every thread reconstructing one cell per chunk, so no
-// cell is reconstructed twice (the segment kernel reconstructs 1120 cells for each 1024-cell y
-// line, 1088 for x).  Thread t of chunk q owns cell r = BS q + t: it computes the face below r and
-// finishes the tendency of cell r-1 (through the stage combination in MODE 2); the face between two
-// chunks takes the lower chunk's top values from an LDS carry.  The periodic face between cells
-// N-1 and 0 is computed last, by the thread of cell N-1, from cell 0's lower values kept in LDS;
-// the tendencies of cells 0 and N-1, which need it, are finished after the last chunk.  The
-// operations are those of k_sweep<ORD, T, DIR, MODE> (whose face 0 comes from the periodic
-// neighbour cell -1, with the values of cell N-1), but the compiler fuses some multiply-adds
-// differently: results agree with it to rounding, not bitwise.
-// ------------------------------------------------------------------------------------------------
-// Where it is used (bits: y FP64, x FP64, y FP32, x FP32; the others use the segment kernel).  The
-// chunk loop costs registers: at 1024x1024x128 WENO-9 the y sweep in FP64 keeps its occupancy and
-// runs 58.1 -> 54.7 ms/step, while x in FP64 (occupancy 4 -> 3) and both FP32 sweeps (7 -> 4-5)
-// ran slower than the 6-9 % of halo reconstructions they save.
-#ifndef PU_PLINE
-#define PU_PLINE 1
-#endif
-template <int DIR> struct LineShape;
-template <> struct LineShape<0> { static constexpr int BX = 64, BY = 4, BS = 64, NA = 4; };
-template <> struct LineShape<1> { static constexpr int BX = 8, BY = 32, BS = 32, NA = 8; };
-
-template <int ORD, class T, int DIR, int MODE>
-__global__ __launch_bounds__(256) void k_sweep_pline(const T *__restrict__ prim, const uint8_t *__restrict__ immf,
-                                                     const uint8_t *__restrict__ any10, T *__restrict__ tend,
-                                                     SweepParams P, FuseArgs<T> FA) {
-  using SH = LineShape<DIR>;
-  constexpr int BX = SH::BX, BY = SH::BY, BS = SH::BS, NA = SH::NA, NT = BX * BY;
-  constexpr int nbs = DIR == 0 ? 1 : BX;
-  __shared__ T bufA0[NT], bufA1[NT], bufA2[NT], bufB0[NT], bufB1[NT];
-  __shared__ T carQ[2][ZW_NQ][NA], carF[2][ZW_NQ][NA];          // by chunk parity
-  __shared__ T firstQ[ZW_NQ][NA], firstF[ZW_NQ][NA];             // cell 0's lower values, face 1's fluxes
-  __shared__ T lastF[ZW_NQ][NA], wrapF[ZW_NQ][NA], wrapU[2][NA]; // faces N-1 and N (= 0)
-
-  const int tx = threadIdx.x, ty = threadIdx.y, lin = ty * BX + tx;
-  const int t = DIR == 0 ? tx : ty, a = DIR == 0 ? ty : tx;
-  unsigned bx, by, bz;
-  swizzled_block(bx, by, bz);
-  const int k = by;
-  const int ia = DIR == 0 ? 0 : bx * BX + tx, ja = DIR == 0 ? bx * BY + ty : 0;   // across position
-  const int N = DIR == 0 ? P.nx : P.ny;
-  const bool across_ok = DIR == 0 ? ja < P.ny : ia < P.nx;
-  const long long sstride = DIR == 0 ? 1 : P.nxh;
-  const long long cstride = DIR == 0 ? 1 : P.nx;
-  const T rd = (T)P.rd;
-  const T hr = (T)P.hy_rc[P.hs + k], ht = (T)P.hy_tc[P.hs + k];
-  const int nchunk = (N + BS - 1) / BS;
-  const int nB = 3 + P.ntr;
-  constexpr int ln = 1 + DIR;
-  constexpr int lt1 = DIR == 0 ? idV : idU;
-  auto field_of = [&](int b, int &l, int &fidx, bool &modify) {
-    modify = b < 2;
-    if (b == 0) { l = lt1; fidx = lt1; }
-    else if (b == 1) { l = idW; fidx = idW; }
-    else if (b == 2) { l = idT; fidx = idT; }
-    else { l = NUM_STATE + 1 + (b - 3); fidx = NUM_STATE + (b - 3); }
-  };
-  auto cell_id = [&](int r) {
-    const int i = DIR == 0 ? r : ia, j = DIR == 0 ? ja : r;
-    return ((long long)k * P.ny + j) * P.nx + i;
-  };
-
-  // tendency of one field of cell `cell`: MODE 1 adds it to `tend`, MODE 2 finishes it through the
-  // stage combination with the cell's stage inputs S
-  constexpr bool PREFETCH = std::is_same<T, double>::value;
-  struct Pre { T t; double u, us; };
-  struct StageIn { double rho, u, v, mult, trr, trt; };
-  auto load_pre = [&](int fidx, long long cell, bool on) {
-    Pre p{T(0), 0., 0.};
-    if (on) {
-      p.t = tend[(long long)fidx * P.n3 + cell];
-      if constexpr (MODE == 2) {
-        p.u = load_cons(FA.U, FA.Ulo, fidx, cell, P.n3);
-        if (FA.C.stage > 0) p.us = load_cons(FA.Us, FA.Uslo, fidx, cell, P.n3);
-      }
-    }
-    return p;
-  };
-  auto load_stage = [&](long long cell, bool on) {
-    StageIn S{0., 0., 0., 0., 0., 0.};
-    if constexpr (MODE == 2) {
-      if (on) stage_inputs(FA, cell, k, S.rho, S.u, S.v, S.mult, S.trr, S.trt);
-    }
-    return S;
-  };
-  auto finish = [&](int fidx, long long cell, T v, const Pre &pf, const StageIn &S) {
-    if constexpr (MODE == 2) {
-      combine_field_v(FA, fidx, cell, (double)(pf.t + v), pf.u, pf.us, S.rho, S.u, S.v, S.mult, S.trr, S.trt);
-    } else {
-      tend[(long long)fidx * P.n3 + cell] = pf.t + v;
-    }
-  };
-
-  for (int q = 0; q < nchunk; q++) {
-    const int r = q * BS + t;
-    const bool valid = across_ok && r < N;
-    const bool carry_in = t == 0 && q > 0, carry_out = t == BS - 1;
-    const bool first = valid && r == 0, second = valid && r == 1, wrapper = valid && r == N - 1;
-    const bool has_face = valid && r >= 1, has_tend = valid && r >= 2;
-    const int pin = (q + 1) & 1, pout = q & 1;
-    const long long cid = cell_id(r), eid = cid - cstride;   // own cell, tendency cell r-1
-    const int i = DIR == 0 ? r : ia, j = DIR == 0 ? ja : r;
-    const long long base = ((long long)(k + P.hp) * P.nyh + (j + P.hp)) * P.nxh + (i + P.ox);
-
-    StencilFlags<ORD> sf;
-    if (valid) sf.setup(immf, base, sstride, any10[cid] != 0, P.weno_all);
-    auto recon = [&](int l, bool modify, T &qL, T &qR) {
-      sf.template recon<T>(prim + (long long)l * P.nh + base, sstride, modify, qL, qR);
-    };
-    const StageIn S = load_stage(eid, has_tend);
-    auto prefetch = [&](int fidx) { return PREFETCH ? load_pre(fidx, eid, has_tend) : Pre{T(0), 0., 0.}; };
-    auto pre_at_use = [&](int fidx, const Pre &p0) { return PREFETCH ? p0 : load_pre(fidx, eid, has_tend); };
-
-    // ---- phase A: density, normal velocity, pressure -> acoustic upwinding ----
-    T qLr = 0, qRr = 0, qLu = 0, qRu = 0, qLp = 0, qRp = 0;
-    const Pre preR = prefetch(idR), preU = prefetch(ln);
-    if (valid) {
-      recon(idR, false, qLr, qRr);
-      recon(ln, false, qLu, qRu);
-      recon(idP, true, qLp, qRp);
-    }
-    bufA0[lin] = qRr; bufA1[lin] = qRu; bufA2[lin] = qRp;
-    if (first) { firstQ[0][a] = qLr; firstQ[1][a] = qLu; firstQ[2][a] = qLp; }
-    __syncthreads();
-    Upw<T> up{T(0), T(0), true};
-    T Fr = 0, Fu = 0;
-    if (has_face) {
-      T rL, uL, pL;
-      if (carry_in) { rL = carQ[pin][0][a]; uL = carQ[pin][1][a]; pL = carQ[pin][2][a]; }
-      else { rL = bufA0[lin - nbs]; uL = bufA1[lin - nbs]; pL = bufA2[lin - nbs]; }
-      up = riemann(rL, uL, pL, qLr, qLu, qLp, hr);
-      Fr = up.ru;
-      Fu = up.ru * (up.left ? uL : qLu) + up.p;
-    }
-    if (wrapper) {
-      const T uR = firstQ[1][a];
-      const Upw<T> w = riemann(qRr, qRu, qRp, firstQ[0][a], uR, firstQ[2][a], hr);
-      wrapU[0][a] = w.ru; wrapU[1][a] = w.left ? T(1) : T(0);
-      wrapF[0][a] = w.ru; wrapF[1][a] = w.ru * (w.left ? qRu : uR) + w.p;
-      lastF[0][a] = Fr; lastF[1][a] = Fu;
-    }
-    bufB0[lin] = Fr; bufB1[lin] = Fu;
-    if (second) { firstF[0][a] = Fr; firstF[1][a] = Fu; }
-    __syncthreads();
-    if (has_tend) {
-      const T Fr0 = carry_in ? carF[pin][0][a] : bufB0[lin - nbs];
-      const T Fu0 = carry_in ? carF[pin][1][a] : bufB1[lin - nbs];
-      finish(idR, eid, -(Fr - Fr0) * rd, pre_at_use(idR, preR), S);
-      finish(ln, eid, -(Fu - Fu0) * rd, pre_at_use(ln, preU), S);
-    }
-    if (carry_out) {
-      carQ[pout][0][a] = qRr; carQ[pout][1][a] = qRu; carQ[pout][2][a] = qRp;
-      carF[pout][0][a] = Fr; carF[pout][1][a] = Fu;
-    }
-
-    // ---- phase B: the advected fields, one at a time ----
-    for (int b = 0; b < nB; b++) {
-      int l, fidx;
-      bool modify;
-      field_of(b, l, fidx, modify);
-      T qL = 0, qR = 0;
-      const Pre pf = prefetch(fidx);
-      if (valid) recon(l, modify, qL, qR);
-      bufA0[lin] = qR;
-      if (first) firstQ[3 + b][a] = qL;
-      __syncthreads();
-      T F = 0;
-      if (has_face) {
-        T vL = carry_in ? carQ[pin][3 + b][a] : bufA0[lin - nbs], vR = qL;
-        if (l == idT) { vL += ht; vR += ht; }
-        F = up.ru * (up.left ? vL : vR);
-      }
-      if (wrapper) {
-        T vL = qR, vR = firstQ[3 + b][a];
-        if (l == idT) { vL += ht; vR += ht; }
-        wrapF[2 + b][a] = wrapU[0][a] * (wrapU[1][a] != T(0) ? vL : vR);
-        lastF[2 + b][a] = F;
-      }
-      bufB0[lin] = F;
-      if (second) firstF[2 + b][a] = F;
-      __syncthreads();
-      if (has_tend) finish(fidx, eid, -(F - (carry_in ? carF[pin][2 + b][a] : bufB0[lin - nbs])) * rd, pre_at_use(fidx, pf), S);
-      if (carry_out) { carQ[pout][3 + b][a] = qR; carF[pout][2 + b][a] = F; }
-    }
-  }
-  __syncthreads();
-
-  // ---- cells 0 (thread t = 0) and N-1 (t = 1): the periodic face closes their tendencies ----
-  if (across_ok && t < 2) {
-    const int slot_lo = t == 0 ? 0 : 1;                 // which face arrays bound the cell
-    const long long cell = cell_id(t == 0 ? 0 : N - 1);
-    const StageIn S = load_stage(cell, true);
-    auto close = [&](int slot, int fidx) {
-      const T Fhi = slot_lo == 0 ? firstF[slot][a] : wrapF[slot][a];
-      const T Flo = slot_lo == 0 ? wrapF[slot][a] : lastF[slot][a];
-      finish(fidx, cell, -(Fhi - Flo) * rd, load_pre(fidx, cell, true), S);
-    };
-    close(0, idR);
-    close(1, ln);
-    for (int b = 0; b < nB; b++) {
-      int l, fidx;
-      bool modify;
-      field_of(b, l, fidx, modify);
-      close(2 + b, fidx);
-    }
   }
 }
 
@@ -780,7 +484,12 @@ static int sweep_one(pu_ctx *ctx, hipStream_t s, int mode, int stage, double dt,
       constexpr int D = DIR <= 1 ? DIR : 0;
       dim3 lb(LineShape<D>::BX, LineShape<D>::BY, 1);
       dim3 lg(DIR == 0 ? (unsigned)((g.ny + lb.y - 1) / lb.y) : (unsigned)((g.nx + lb.x - 1) / lb.x), (unsigned)g.nz, 1);
-      if (mode == 2) {
+      if constexpr (std::is_same<T, float>::value && DIR == 1) {
+        // FP32 y lines: their own translation unit (sweep_pline_f32.hip); y never combines the stage
+        if (mode != 1) return fail(PU_ERR_ARG, "the y sweep does not combine the stage");
+        const int rc = launch_pline_y_f32(ORD, lg, lb, s, prim, ctx->d_imm_flag, ctx->d_any10, tend, P, FA);
+        if (rc) return rc;
+      } else if (mode == 2) {
         FA.U = (T *)ctx->d_U; FA.Ulo = (T *)ctx->d_Ulo; FA.Us = (T *)ctx->d_Us; FA.Uslo = (T *)ctx->d_Uslo;
         FA.C = combine_params(ctx);
         FA.C.stage = stage; FA.C.dt = dt; FA.C.nrelax = nrelax;

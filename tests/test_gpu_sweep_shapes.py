@@ -3,10 +3,10 @@ against the segment kernels (forced by the environment variable PORTURB_SEGMENT_
 some directions by PORTURB_SEGMENT_DIRS, when the context is created):
   * k_sweep_zwall (z between solid walls) agrees bitwise -- tendencies and whole time steps, FP64
     and FP32, columns shorter than one chunk of 32 cells, exact multiples and ragged last chunks;
-  * k_sweep_pline (periodic y lines, FP64) agrees to rounding (the compiler fuses some
-    multiply-adds differently): tendencies within 1e-13 of each field's largest, states after two
-    steps within 1e-12 relative -- lines shorter than one chunk, exact multiples, a last chunk
-    holding only the periodic end cell and a two-cell line.  A wrong carry, wrap or ownership
+  * k_sweep_pline (periodic y lines) agrees to rounding (the compiler fuses some multiply-adds
+    differently): tendencies within 1e-13 of each field's largest, states after two steps within
+    1e-12 relative in FP64, both within 1e-5 in FP32 -- lines shorter than one chunk, exact
+    multiples, a last chunk holding only the periodic end cell and two-cell lines.  A wrong carry, wrap or ownership
     would differ at the size of the tendencies themselves."""
 import os
 
@@ -70,6 +70,8 @@ def test_zwall_sweep_matches_segment_sweep_bitwise(case):
 
 YCASES = [
     dict(ord=9, nx=20, ny=70, nz=12),
+    dict(ord=9, nx=20, ny=70, nz=12, precision=pa.PU_FP32),
+    dict(ord=3, nx=16, ny=2, nz=10, precision=pa.PU_FP32),
     dict(ord=5, nx=24, ny=64, nz=10),
     dict(ord=7, nx=16, ny=65, nz=10, bc_z=pa.PU_BC_PERIODIC),
     dict(ord=9, nx=18, ny=20, nz=12),
@@ -82,9 +84,10 @@ def test_periodic_line_sweep_matches_segment_sweep(case):
     cfg, st, imm = setup(case)
     t1, s1 = run(cfg, st, imm, None)
     t2, s2 = run(cfg, st, imm, "y")
+    fp32 = case.get("precision") == pa.PU_FP32
     scale = np.maximum(np.abs(t2).reshape(t2.shape[0], -1).max(axis=1), 1e-300)
     err = np.abs(t1 - t2).reshape(t1.shape[0], -1).max(axis=1) / scale
-    assert err.max() < 1e-13, err
+    assert err.max() < (1e-5 if fp32 else 1e-13), err
     for f in range(s1.shape[0]):
         d = np.abs(s1[f] - s2[f]).max() / max(np.abs(s2[f]).max(), 1e-300)
-        assert d < 1e-12, (f, d)
+        assert d < (1e-5 if fp32 else 1e-12), (f, d)
