@@ -255,9 +255,11 @@ int oc_sgs_features(const double *les, int nx, int ny, int nz, const float *fmul
  * A cell falls back to the eddy-viscosity closure (bit cleared) when
  *   - it is in the first or last level, or any of its 27 neighbours has immersed proportion > 0;
  *   - any normalised input exceeds M->xmax in magnitude (out of the training range);
- *   - momentum: the predicted dissipation eps = -tau_ij S_ij is negative, or exceeds
- *     nu_max * 2 S_ij S_ij (an effective viscosity above the closure's stability cap nu_max);
- *   - heat: chi = -q_j dtheta/dx_j is negative or exceeds nu_max |grad theta|^2.
+ *   - momentum: the resolved strain is exactly zero (nothing bounds the stresses), or the
+ *     predicted dissipation eps = -tau_ij S_ij is negative, or exceeds nu_max * 2 S_ij S_ij (an
+ *     effective viscosity above the closure's stability cap nu_max);
+ *   - heat: the resolved theta gradient is exactly zero, or chi = -q_j dtheta/dx_j is negative or
+ *     exceeds nu_max |grad theta|^2.
  * tau: [9][nz][ny][nx] (tau11 tau12 tau13 tau22 tau23 tau33 q1 q2 q3, physical units). */
 void oc_sgs_infer(const oc_sgs_model *M, const double *les, const double *imm, int nx, int ny, int nz, double dx,
                   double dy, double dz, double nu_max, int bf16, float *tau, unsigned char *flags) {
@@ -294,8 +296,8 @@ void oc_sgs_infer(const oc_sgs_model *M, const double *les, const double *imm, i
       double ss = s11 * s11 + s22 * s22 + s33 * s33 + 2 * (s12 * s12 + s13 * s13 + s23 * s23);
       double chi = -(t[6] * tx + t[7] * ty + t[8] * tz);
       double gg = tx * tx + ty * ty + tz * tz;
-      if (eps >= 0 && eps <= nu_max * 2 * ss) fl |= 1;
-      if (chi >= 0 && chi <= nu_max * gg) fl |= 2;
+      if (ss > 0 && eps >= 0 && eps <= nu_max * 2 * ss) fl |= 1;
+      if (gg > 0 && chi >= 0 && chi <= nu_max * gg) fl |= 2;
     }
     flags[id] = fl;
   }

@@ -596,8 +596,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const double ss = s11 * s11 + s22 * s22 + s33 * s33 + 2 * (s12 * s12 + s13 * s13 + s23 * s23);
         const double chi = -(t[6] * tx + t[7] * ty + t[8] * tz);
         const double gg = tx * tx + ty * ty + tz * tz;
-        if (eps >= 0 && eps <= A.nu_max * 2 * ss) fl |= 1;
-        if (chi >= 0 && chi <= A.nu_max * gg) fl |= 2;
+        // a zero strain (gradient) bounds nothing: any stress (flux) gives eps = 0 (chi = 0)
+        if (ss > 0 && eps >= 0 && eps <= A.nu_max * 2 * ss) fl |= 1;
+        if (gg > 0 && chi >= 0 && chi <= A.nu_max * gg) fl |= 2;
       }
       A.flags[id] = fl;
     }

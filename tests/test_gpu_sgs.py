@@ -295,3 +295,33 @@ def test_driver_with_learned_closure(tmp_path):
         assert np.all(np.isfinite(sa[f])), f
     ua, ub = sa["uvel"].mean(axis=(1, 2)), sb["uvel"].mean(axis=(1, 2))
     assert np.abs(ua - ub).max() <= 0.02 * np.abs(ub).max()
+
+
+def test_trained_network_flags_and_outputs_match_oracle():
+    """The shipped network (porturb_amd/data/sgs_abl_urban_dx2.bin) on a 96x96x32 piece of the
+    case it was trained for (neutral urban ABL at 2 m, seeded state with buildings): outputs to the
+    BF16 bounds above, and each fallback bit (momentum, heat) set on the same cells as the oracle
+    for >= 99 % of the cells, with the same fraction of learned cells to 0.5 %."""
+    import os
+    path = os.path.join(os.path.dirname(pa.__file__), "data", "sgs_abl_urban_dx2.bin")
+    gm = pa.Mlp.load(path)
+    fs, xmax, om, os_ = gm.get_normalisation()
+    model = o.SgsModel(gm.get_params(), fs, xmax, om, os_)
+    cfg = cases.abl_urban(96, 96, 32, 2.0, ord=9)
+    cs = pa.CouplerState(cfg)
+    pa.case_init(cfg, cfg["case_id"], 5, cs)
+    st, imm = cs.download(), cs.download_immersed()
+    dt = 0.02
+    _, tau, fl = run_learned(cfg, st, imm, gm, dt)
+    ref = st.copy()
+    rtau, rfl = o.les_apply_learned(cfg, ref, imm, dt, model)
+    for bit in (1, 2):
+        g, r = (fl & bit) != 0, (rfl & bit) != 0
+        assert (g == r).mean() >= 0.99, (bit, (g == r).mean())
+        assert abs(g.mean() - r.mean()) < 5e-3, (bit, g.mean(), r.mean())
+        assert r.mean() > 0.05, (bit, r.mean())
+    for f in range(9):
+        s = max(np.abs(rtau[f]).max(), 1e-30)
+        err = np.abs(tau[f] - rtau[f])
+        assert np.median(err) <= 1e-4 * s and err.max() <= 2e-2 * s, f
+    gm.destroy()

@@ -252,6 +252,15 @@ def test_fallback_rule_known_answers():
     # out of range inputs -> fallback for both
     _, fl4 = o.sgs_infer(const_model([0, 0, -0.01, 0, 0, 0, 0, 0, -0.001], xmax=0.1), les, imm, 1., 1., 1., nu_max)
     assert np.all(fl4 == 0)
+    # a zero theta gradient bounds no heat flux -> heat fallback; a zero strain likewise for momentum
+    flat = halo_box([lambda k, j, i: 1 + 0 * i, lambda k, j, i: dudz * k, lambda k, j, i: 0 * i,
+                     lambda k, j, i: 0 * i, lambda k, j, i: 300 + 0 * i, lambda k, j, i: 0 * i], nz, ny, nx)
+    _, fl5 = o.sgs_infer(const_model([0, 0, -0.01, 0, 0, 0, 0, 0, -0.001]), flat, imm, 1., 1., 1., nu_max)
+    assert np.all(fl5[inner & ~near] == 1)
+    still = halo_box([lambda k, j, i: 1 + 0 * i, lambda k, j, i: 0 * i, lambda k, j, i: 0 * i,
+                      lambda k, j, i: 0 * i, lambda k, j, i: 300 + 0.01 * k, lambda k, j, i: 0 * i], nz, ny, nx)
+    _, fl6 = o.sgs_infer(const_model([0, 0, 0, 0, 0, 0, 0, 0, -0.001]), still, imm, 1., 1., 1., nu_max)
+    assert np.all(fl6[inner & ~near] == 2)
 
 
 def shear_case(n=8, nz=10):
@@ -281,7 +290,8 @@ def test_learned_apply_zero_stress_removes_interior_momentum_flux():
     o.les_apply(cfg, a, imm, 0.01)
     _, fl = o.les_apply_learned(cfg, b, imm, 0.01, const_model(np.zeros(9)))
     nz = cfg["nz"]
-    assert np.all(fl[1:nz - 1] == 3) and np.all(fl[0] == 0) and np.all(fl[nz - 1] == 0)
+    # theta is uniform here: no gradient bounds a heat flux, so only the momentum bit is set
+    assert np.all(fl[1:nz - 1] == 1) and np.all(fl[0] == 0) and np.all(fl[nz - 1] == 0)
     # the plain closure diffuses the curved profile on every level; learned faces (both cells
     # flagged) carry zero flux, so u is unchanged on levels 2..nz-3, and level 0 keeps the plain update
     assert np.abs(a[1] - c[1])[1].min() > 1e-6 and np.abs(a[1] - c[1])[3].min() > 1e-6
