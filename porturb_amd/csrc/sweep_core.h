@@ -70,7 +70,12 @@ template <class T, int S> __device__ __forceinline__ void excl_products(const T 
 //    (D >= 1e-20*tot^2 keeps products of S-1 factors far from underflow in double).
 //  * map_rs returns A_m/B_m; the renormalised mapped weights are A_m*Q_m / sum A_k*Q_k with
 //    Q_m = prod_{j!=m} B_j, and the final division is applied once to the face value.
-template <int ORD>
+// Build switch for the flat-stencil branch of the limiters (1: on).
+#ifndef PU_FLAT_SHORTCUT
+#define PU_FLAT_SHORTCUT 1
+#endif
+
+template <int ORD, bool FLAT>
 __device__ __forceinline__ void weno_edges_fp64(const double *s, bool do_map, bool imm_L, bool imm_R,
                                                 double &qL, double &qR) {
   using O = pu_weno::Ops<ORD>;
@@ -80,6 +85,17 @@ __device__ __forceinline__ void weno_edges_fp64(const double *s, bool do_map, bo
   double tot = TV[0];
 #pragma unroll
   for (int m = 1; m < S; m++) tot += TV[m];
+  // Flat stencil (tot <= 1e-20, the reference's convexify guard): every D_m rounds to 1e-20, the
+  // weights are the ideal ones and the map leaves them unchanged, so the limited edges are the
+  // linear reconstruction to rounding.  Decided per cell (the result depends on the cell's own
+  // stencil only); a wave whose cells are all flat skips the limiter.
+  if constexpr (FLAT && PU_FLAT_SHORTCUT) {
+    if (tot <= 1.e-20) {
+      qL = O::template linL<double>(s);
+      qR = O::template linR<double>(s);
+      return;
+    }
+  }
   const double escale = tot > 1.e-20 ? 1.e-20 * (tot * tot) : 1.e-20;
   if (imm_L || imm_R) {
     double mx = TV[0];
@@ -131,16 +147,27 @@ __device__ __forceinline__ void weno_edges_fp64(const double *s, bool do_map, bo
   }
 }
 
-template <int ORD, class T>
+// FLAT: flat stencils take the linear reconstruction (see weno_edges_fp64)
+template <int ORD, class T, bool FLAT = true>
 __device__ __forceinline__ void weno_edges(const T *s, bool do_map, bool imm_L, bool imm_R, T &qL, T &qR) {
   using O = pu_weno::Ops<ORD>;
   constexpr int S = O::S;
   if constexpr (std::is_same<T, double>::value) {
-    weno_edges_fp64<ORD>(s, do_map, imm_L, imm_R, qL, qR);
+    weno_edges_fp64<ORD, FLAT>(s, do_map, imm_L, imm_R, qL, qR);
     return;
   } else {
   T TV[S];
   O::template tv<T>(s, TV);
+  if constexpr (FLAT && PU_FLAT_SHORTCUT) {                              // flat stencils: as in FP64 above
+    T tot = TV[0];
+#pragma unroll
+    for (int m = 1; m < S; m++) tot += TV[m];
+    if (tot <= T(1.e-20)) {
+      qL = O::template linL<T>(s);
+      qR = O::template linR<T>(s);
+      return;
+    }
+  }
   convexify<T, S>(TV);
   if (imm_L || imm_R) {
     T mx = TV[0];
@@ -234,7 +261,7 @@ template <int ORD> struct StencilFlags {
   }
 
   // both interface values of the cell whose field value sits at p (stride sstride along the sweep)
-  template <class T>
+  template <class T, bool FLAT = true>
   __device__ __forceinline__ void recon(const T *p, long long sstride, bool modify, T &qL, T &qR) const {
     p -= (long long)hs * sstride;
     T s[ORD];
@@ -246,7 +273,7 @@ template <int ORD> struct StencilFlags {
       for (int q = 0; q < ORD; q++) s[q] = p[(long long)q * sstride];
     }
     if (use_weno) {
-      weno_edges<ORD, T>(s, do_map, imm_L, imm_R, qL, qR);
+      weno_edges<ORD, T, FLAT>(s, do_map, imm_L, imm_R, qL, qR);
     } else {
       qL = pu_weno::Ops<ORD>::template linL<T>(s);
       qR = pu_weno::Ops<ORD>::template linR<T>(s);
