@@ -64,192 +64,6 @@ __global__ __launch_bounds__(256) void k_prep(const ST *__restrict__ U, const ST
 }
 
 // ------------------------------------------------------------------------------------------------
-// sweep along one direction: reconstruction + Riemann + divergence, face states through LDS.
-// A 256-thread block covers line segments of BS-2 cells along the sweep plus one neighbour cell at
-// each end (the cell whose reconstruction the segment's end faces need); x-contiguous threads keep
-// the loads of y and z sweeps coalesced.  Whole-line blocks (no neighbour cells) were measured
-// slower: 1024-thread blocks leave one block per CU, idle at every barrier.
-// ------------------------------------------------------------------------------------------------
-// Block shapes (BX across, BS along the sweep; BY = BS for y/z).  PU_SEG_{Y,Z}_{BX,BS} override
-// them at compile time for tuning builds.
-#ifndef PU_SEG_Y_BX
-#define PU_SEG_Y_BX 8
-#endif
-#ifndef PU_SEG_Y_BS
-#define PU_SEG_Y_BS 32
-#endif
-#ifndef PU_SEG_Z_BX
-#define PU_SEG_Z_BX 8
-#endif
-#ifndef PU_SEG_Z_BS
-#define PU_SEG_Z_BS 32
-#endif
-template <int DIR> struct SegShape;
-template <> struct SegShape<0> { static constexpr int BX = 64, BY = 4, BS = 64; };
-template <> struct SegShape<1> { static constexpr int BX = PU_SEG_Y_BX, BY = PU_SEG_Y_BS, BS = PU_SEG_Y_BS; };
-template <> struct SegShape<2> { static constexpr int BX = PU_SEG_Z_BX, BY = PU_SEG_Z_BS, BS = PU_SEG_Z_BS; };
-
-// MODE: 0 = first sweep of the stage (writes the tendencies), 1 = adds to them, 2 = last sweep:
-// each cell's final tendency goes straight into the stage combination (combine_field_v) instead of
-// back to `tend`.  The stage runs z (0), y (1), x (2): the last sweep is the one whose rows are
-// 64 cells wide, so the stage-state reads and writes of the combination are coalesced.
-template <int ORD, class T, int DIR, int MODE>
-__global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY) void k_sweep(const T *__restrict__ prim, const uint8_t *__restrict__ immf,
-                                               const uint8_t *__restrict__ any10, T *__restrict__ tend,
-                                               SweepParams P, FuseArgs<T> FA) {
-  constexpr int hs = (ORD - 1) / 2;
-  constexpr int BX = SegShape<DIR>::BX, BY = SegShape<DIR>::BY, BS = SegShape<DIR>::BS;
-  constexpr int NC = BS - 2;
-  constexpr int NT = BX * BY;
-  __shared__ T bufA0[NT], bufA1[NT], bufA2[NT], bufB0[NT], bufB1[NT];
-
-  const int tx = threadIdx.x, ty = threadIdx.y, lin = ty * BX + tx;
-  const int t = DIR == 0 ? tx : ty;
-  constexpr int nbs = DIR == 0 ? 1 : BX;
-  unsigned bx, by, bz;
-  swizzled_block(bx, by, bz);
-  int i, j, k, c, nal;
-  if (DIR == 0) { c = bx * NC - 1 + tx; i = c; j = by * BY + ty; k = bz; nal = P.nx; }
-  else if (DIR == 1) { i = bx * BX + tx; c = by * NC - 1 + ty; j = c; k = bz; nal = P.ny; }
-  else { i = bx * BX + tx; j = by; c = bz * NC - 1 + ty; k = c; nal = P.nz; }
-  const bool across_ok = DIR == 0 ? (j < P.ny) : (i < P.nx);
-  const bool in_range = across_ok && c >= -1 && c <= nal;
-  const bool periodic = P.bc_along == PU_BC_PERIODIC;
-  const bool ghost = c < 0 || c >= nal;
-  const bool do_recon = in_range && (!ghost || periodic);
-  const bool has_face = in_range && t >= 1 && c >= 0;
-  const bool has_tend = in_range && t >= 1 && t <= NC && c < nal;
-  const bool bface_lo = !periodic && c == 0;
-  const bool bface_hi = !periodic && c == nal;
-  const bool wallz = DIR == 2 && P.wall_z;
-
-  const long long sstride = DIR == 0 ? 1 : (DIR == 1 ? P.nxh : P.nxh * P.nyh);
-  const long long base = ((long long)(k + P.hp) * P.nyh + (j + P.hp)) * P.nxh + (i + P.ox);
-
-  StencilFlags<ORD> sf;
-  if (do_recon) {
-    // A periodic neighbour cell past either end is the wrapped interior cell: its flags come from
-    // that cell's own position (the z halos of the flag array hold "immersed", not a periodic copy,
-    // dynamics_rk.h:1031-1034; the reference takes that face state from the wrapped cell's
-    // reconstruction through edge_exchange, :890-908).
-    const int cw = wrapi(c, nal);
-    int ci = i, cj = j, ck = k;
-    if (DIR == 0) ci = cw; else if (DIR == 1) cj = cw; else ck = cw;
-    sf.setup(immf, base + (long long)(cw - c) * sstride, sstride, any10[((long long)ck * P.ny + cj) * P.nx + ci] != 0,
-             P.weno_all);
-  }
-  auto recon = [&](int l, bool modify, T &qL, T &qR) {
-    // the FP32 x sweep, which also combines the stage, skips the flat-stencil branch: its registers
-    // would cost two waves of occupancy (71 -> 84 VGPRs)
-    sf.template recon<T, !(std::is_same<T, float>::value && DIR == 0)>(prim + (long long)l * P.nh + base, sstride,
-                                                                      modify, qL, qR);
-  };
-
-  const long long cid = ((long long)k * P.ny + j) * P.nx + i;
-  const T rd = (T)P.rd;
-  double f_rho = 0, f_u = 0, f_v = 0, f_mult = 0, f_trr = 0, f_trt = 0;
-  if constexpr (MODE == 2) {
-    if (has_tend) stage_inputs(FA, cid, k, f_rho, f_u, f_v, f_mult, f_trr, f_trt);
-  }
-  // FP64: the values a cell's finished tendency is combined with are loaded before the field's
-  // reconstruction, so their latency hides behind it (measured 3.5 ms/step faster at cfg 2).  FP32:
-  // loaded when needed -- the extra registers cost the FP32 kernel one wave of occupancy (slower).
-  constexpr bool PREFETCH = std::is_same<T, double>::value;
-  struct Pre { T t; double u, us; };
-  auto load_pre = [&](int fidx) {
-    Pre p{T(0), 0., 0.};
-    if (has_tend) {
-      if constexpr (MODE >= 1) p.t = tend[(long long)fidx * P.n3 + cid];
-      if constexpr (MODE == 2) {
-        p.u = load_cons(FA.U, FA.Ulo, fidx, cid, P.n3);
-        if (FA.C.stage > 0) p.us = load_cons(FA.Us, FA.Uslo, fidx, cid, P.n3);
-      }
-    }
-    return p;
-  };
-  auto prefetch = [&](int fidx) { return PREFETCH ? load_pre(fidx) : Pre{T(0), 0., 0.}; };
-  auto accumulate = [&](int fidx, T v, const Pre &pf0) {
-    T *tp = tend + (long long)fidx * P.n3 + cid;
-    const Pre pf = PREFETCH ? pf0 : load_pre(fidx);
-    if constexpr (MODE == 2) {
-      const T tsum = pf.t + v;
-      combine_field_v(FA, fidx, cid, (double)tsum, pf.u, pf.us, f_rho, f_u, f_v, f_mult, f_trr, f_trt);
-    } else if constexpr (MODE == 1) {
-      *tp = pf.t + v;
-    } else {
-      *tp = v;
-    }
-  };
-
-  // hydrostatic add-back: cell values for x/y faces, edge values for z faces (:642-645, :686-689)
-  T hr = 0, ht = 0;
-  if (has_face) {
-    if (DIR == 2) { hr = (T)P.hy_re[c]; ht = (T)P.hy_te[c]; }
-    else { hr = (T)P.hy_rc[hs + k]; ht = (T)P.hy_tc[hs + k]; }
-  }
-
-  // ---- phase A: density, normal velocity, pressure -> acoustic upwinding ----
-  constexpr int ln = 1 + DIR;
-  T qLr = 0, qRr = 0, qLu = 0, qRu = 0, qLp = 0, qRp = 0;
-  const Pre preR = prefetch(idR), preU = prefetch(ln);
-  if (do_recon) {
-    recon(idR, false, qLr, qRr);
-    recon(ln, false, qLu, qRu);
-    recon(idP, true, qLp, qRp);
-  }
-  bufA0[lin] = qRr; bufA1[lin] = qRu; bufA2[lin] = qRp;
-  __syncthreads();
-  Upw<T> up{T(0), T(0), true};
-  T Fr = 0, Fu = 0;
-  if (has_face) {
-    T rL = bufA0[lin - nbs], uL = bufA1[lin - nbs], pL = bufA2[lin - nbs];
-    T rR = qLr, uR = qLu, pR = qLp;
-    if (bface_lo) { rL = rR; uL = uR; pL = pR; }
-    if (bface_hi) { rR = rL; uR = uL; pR = pL; }
-    if (wallz && (bface_lo || bface_hi)) { uL = 0; uR = 0; }     // W = 0 on both sides at walls
-    up = riemann(rL, uL, pL, rR, uR, pR, hr);
-    Fr = up.ru;
-    Fu = up.ru * (up.left ? uL : uR) + up.p;
-  }
-  bufB0[lin] = Fr; bufB1[lin] = Fu;
-  __syncthreads();
-  if (has_tend) {
-    accumulate(idR, -(bufB0[lin + nbs] - bufB0[lin]) * rd, preR);
-    accumulate(ln, -(bufB1[lin + nbs] - bufB1[lin]) * rd, preU);
-  }
-
-  // ---- phase B: the advected fields, one at a time ----
-  constexpr int lt1 = DIR == 0 ? idV : idU;
-  constexpr int lt2 = DIR == 2 ? idV : idW;
-  const int nB = 3 + P.ntr;
-  for (int b = 0; b < nB; b++) {
-    int l, fidx;
-    bool modify = false;
-    if (b == 0) { l = lt1; fidx = lt1; modify = true; }
-    else if (b == 1) { l = lt2; fidx = lt2; modify = true; }
-    else if (b == 2) { l = idT; fidx = idT; }
-    else { l = NUM_STATE + 1 + (b - 3); fidx = NUM_STATE + (b - 3); }
-    T qL = 0, qR = 0;
-    const Pre pf = prefetch(fidx);
-    if (do_recon) recon(l, modify, qL, qR);
-    bufA0[lin] = qR;
-    __syncthreads();
-    T F = 0;
-    if (has_face) {
-      T vL = bufA0[lin - nbs], vR = qL;
-      if (bface_lo) vL = vR;
-      if (bface_hi) vR = vL;
-      if (wallz && (bface_lo || bface_hi) && l == idT) { vL = 0; vR = 0; }
-      if (l == idT) { vL += ht; vR += ht; }
-      F = up.ru * (up.left ? vL : vR);
-    }
-    bufB0[lin] = F;
-    __syncthreads();
-    if (has_tend) accumulate(fidx, -(bufB0[lin + nbs] - bufB0[lin]) * rd, pf);
-  }
-}
-
-// ------------------------------------------------------------------------------------------------
 // coupler <-> dynamics conversions (dynamics_rk.h:1251-1352), with the first relaxation fused
 // ------------------------------------------------------------------------------------------------
 struct ConvParams {
@@ -516,7 +330,13 @@ static int sweep_one(pu_ctx *ctx, hipStream_t s, int mode, int stage, double dt,
     FA.U = (T *)ctx->d_U; FA.Ulo = (T *)ctx->d_Ulo; FA.Us = (T *)ctx->d_Us; FA.Uslo = (T *)ctx->d_Uslo;
     FA.C = combine_params(ctx);
     FA.C.stage = stage; FA.C.dt = dt; FA.C.nrelax = nrelax;
-    hipLaunchKernelGGL((k_sweep<ORD, T, DIR, 2>), grd, blk, 0, s, prim, ctx->d_imm_flag, ctx->d_any10, tend, P, FA);
+    if constexpr (std::is_same<T, float>::value && DIR == 0)
+      return launch_seg_x_f32(ORD, 2, grd, blk, s, prim, ctx->d_imm_flag, ctx->d_any10, tend, P, FA);
+    else
+      hipLaunchKernelGGL((k_sweep<ORD, T, DIR, 2>), grd, blk, 0, s, prim, ctx->d_imm_flag, ctx->d_any10, tend, P, FA);
+  } else if constexpr (std::is_same<T, float>::value && DIR == 0) {
+    // FP32 x segments: their own translation unit (sweep_seg_x_f32.hip)
+    return launch_seg_x_f32(ORD, mode, grd, blk, s, prim, ctx->d_imm_flag, ctx->d_any10, tend, P, FA);
   } else if (mode == 1) {
     hipLaunchKernelGGL((k_sweep<ORD, T, DIR, 1>), grd, blk, 0, s, prim, ctx->d_imm_flag, ctx->d_any10, tend, P, FA);
   } else {
