@@ -77,6 +77,37 @@ def test_tendencies_match_oracle(case):
     assert err.max() < TOL.get(str(case), 1e-11), err
 
 
+# A passive tracer whose mixing ratio is 1e-13 * (2 + smooth noise): every stencil's total
+# variation is far below the limiter's 1e-20 guard, so the GPU limiters take their flat-stencil
+# branch (the linear reconstruction) for it while the oracle runs the full limiter with the guard.
+# Measured against the tracer's own tendencies the two agree to rounding in FP64; exchanging the
+# branch's left and right edges, or taking one sub-stencil instead of the linear combination, would
+# differ at the size of the tracer's own variation (O(1) here).  FP32 is held to 1e-4 for the tracer.
+@pytest.mark.parametrize("precision", [pa.PU_FP64, pa.PU_FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("ord_", (3, 5, 7, 9, 11))
+def test_flat_tracer_stencils_match_oracle(ord_, precision):
+    cfg = small_config(nx=24, ny=20, nz=16, ord=ord_, tracers=[WV, TKE, P1], precision=precision)
+    st, imm = random_state(cfg, seed=23, buildings=3)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(st.shape[1:])
+    for ax in range(3):
+        f = 0.25 * np.roll(f, 1, ax) + 0.5 * f + 0.25 * np.roll(f, -1, ax)
+    ip = 5 + 2
+    st[ip] = st[0] * 1e-13 * np.clip(2.0 + f, 0.5, None)
+    print("flat tracer, ord", ord_, "precision", precision, flush=True)
+    d, cs = gpu_setup(cfg, st, imm)
+    g = d.tendencies(cs)
+    cfg64 = dict(cfg, precision=pa.PU_FP64)
+    o = oracle_py.Dycore(cfg64, st, imm).tendencies(st)
+    err = np.abs(g - o).reshape(g.shape[0], -1).max(axis=1) / tend_scale(o)
+    print("per-field error", err, flush=True)
+    assert np.abs(o[ip]).max() > 0
+    if precision == pa.PU_FP64:
+        assert err.max() < 1e-11, err
+    else:
+        assert err[ip] < 1e-4, err
+
+
 @pytest.mark.parametrize("ord_", (5, 9))
 def test_hydrostatics_and_immersed_masks_match(ord_):
     cfg = small_config(nx=24, ny=22, nz=16, ord=ord_, tracers=[WV, P1])
