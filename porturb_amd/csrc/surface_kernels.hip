@@ -47,6 +47,12 @@ __global__ __launch_bounds__(256) void k_surface_fluxes(double *uu, double *vv, 
   const long long id = (long long)k * nxy + p;
   const long long nxh = P.nx + 2, nyh = P.ny + 2;
   const long long c = ((long long)(k + 1) * nyh + (j + 1)) * nxh + (i + 1);
+  // a cell without an immersed face neighbour has zero tendencies: it is left as it is (u + dt*0
+  // is u), so most of the domain reads only the immersed proportions and writes nothing
+  const long long sz = nxh * nyh;
+  if (!(P.prop[c - 1] > 0 || P.prop[c + 1] > 0 || P.prop[c - nxh] > 0 || P.prop[c + nxh] > 0 ||
+        P.prop[c - sz] > 0 || P.prop[c + sz] > 0))
+    return;
   const double u = uu[id], v = vv[id], w = ww[id], T = tt[id];
   const double vk = 0.40;
   double tu = 0, tv = 0, tw = 0, tT = 0;

@@ -248,3 +248,6 @@ def test_surface_fluxes_match_oracle():
     assert np.abs(g[1:5] - st[1:5]).max() > 0
     nrm = rel_norms(g, ref)
     assert nrm.max() < 1e-13, nrm
+    # cells away from every immersed face (the kernel skips them) keep their values exactly
+    far = np.all(ref[1:5] == st[1:5], axis=0)
+    assert far.sum() > far.size // 4 and np.array_equal(g[1:5][:, far], st[1:5][:, far])
