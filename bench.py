@@ -185,6 +185,10 @@ def main():
                       "path": "C++ Coupler + modules",
                       "closure": ("learned (bf16 matrix cores) + tke fallback" if learned else "tke") if les else "none",
                       "tracers": cfg["num_tracers"],
+                      # the timed steps follow the warm-up steps from the seeded case initialisation:
+                      # the air above the boundary layer is still uniform, where the limiters take
+                      # their flat-stencil branch (the linear reconstruction, equal to rounding)
+                      "state": "case initialisation + %d warm-up steps" % args.warmup,
                       "step": ("dycore.time_step + apply_surface_fluxes + " +
                                ("windmills.apply + " if turbines else "") + "les_closure.apply") if les else "dycore.time_step"},
            "roofline": roof}
