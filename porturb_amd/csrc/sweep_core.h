@@ -1,7 +1,9 @@
-// sweep_core.h -- device code shared by the sweep kernels of dycore_kernels.hip and by
-// sweep_zwall_f32.hip (the FP32 whole-column z sweep, compiled in its own translation unit without
-// loop-invariant code motion of machine code: hoisting the limiter's constants out of the chunk
-// loop costs that kernel registers and a wave of occupancy).
+// sweep_core.h -- the sweep kernels (WENO limiter, Riemann flux, flux divergence and the fused
+// stage combination): the segment sweep k_sweep, the whole-column z sweep k_sweep_zwall and the
+// whole-line sweep k_sweep_pline.  dycore_kernels.hip launches them; the FP32 z, y and x instances
+// are compiled in translation units of their own (sweep_zwall_f32.hip, sweep_pline_f32.hip,
+// sweep_seg_x_f32.hip) without loop-invariant code motion of machine code: hoisting the limiter's
+// constants out of the loops costs those kernels registers and waves of occupancy.
 #pragma once
 #include "pu_common.h"
 #include "weno_ops.h"
