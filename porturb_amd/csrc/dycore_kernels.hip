@@ -27,6 +27,18 @@ struct PrepParams {
 // ------------------------------------------------------------------------------------------------
 // prep: primitive fields with halos (cross-shaped: the halo corners are never read)
 // ------------------------------------------------------------------------------------------------
+// Stores of the primitive fields, non-temporal (PU_NT_STORE=0: ordinary stores): the fields are far
+// larger than the caches and the next kernel reads them from HBM either way.  Measured at
+// 1024x1024x128 WENO-9 FP64: the step 186.4 -> 185.2 ms.
+#ifndef PU_NT_STORE
+#define PU_NT_STORE 1
+#endif
+template <class ST>
+__device__ __forceinline__ void put_prim(ST *p, ST v) {
+  if (PU_NT_STORE) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 template <class ST>
 __global__ __launch_bounds__(256) void k_prep(const ST *__restrict__ U, const ST *__restrict__ Ulo,
                                               ST *__restrict__ prim, PrepParams P) {
@@ -52,15 +64,14 @@ __global__ __launch_bounds__(256) void k_prep(const ST *__restrict__ U, const ST
   double rt = load_cons(U, Ulo, idT, sid, P.n3);
   int kc = P.hs + ks;
   double r_r = 1. / rho;
-  prim[(long long)idP * P.nh + hid] = (ST)(P.C0 * pow(rt, P.gamma) - P.hy_p[kc]);
-  prim[(long long)idR * P.nh + hid] = (ST)(rho - P.hy_r[kc]);
-  prim[(long long)idU * P.nh + hid] = (ST)((double)U[(long long)idU * P.n3 + sid] * r_r);
-  prim[(long long)idV * P.nh + hid] = (ST)((double)U[(long long)idV * P.n3 + sid] * r_r);
-  prim[(long long)idW * P.nh + hid] = zwall ? (ST)0 : (ST)((double)U[(long long)idW * P.n3 + sid] * r_r);
-  prim[(long long)idT * P.nh + hid] = zwall ? (ST)0 : (ST)(rt * r_r - P.hy_t[kc]);
+  put_prim(prim + (long long)idP * P.nh + hid, (ST)(P.C0 * pow(rt, P.gamma) - P.hy_p[kc]));
+  put_prim(prim + (long long)idR * P.nh + hid, (ST)(rho - P.hy_r[kc]));
+  put_prim(prim + (long long)idU * P.nh + hid, (ST)((double)U[(long long)idU * P.n3 + sid] * r_r));
+  put_prim(prim + (long long)idV * P.nh + hid, (ST)((double)U[(long long)idV * P.n3 + sid] * r_r));
+  put_prim(prim + (long long)idW * P.nh + hid, zwall ? (ST)0 : (ST)((double)U[(long long)idW * P.n3 + sid] * r_r));
+  put_prim(prim + (long long)idT * P.nh + hid, zwall ? (ST)0 : (ST)(rt * r_r - P.hy_t[kc]));
   for (int tr = 0; tr < P.ntr; tr++)
-    prim[(long long)(NUM_STATE + 1 + tr) * P.nh + hid] =
-        (ST)((double)U[(long long)(NUM_STATE + tr) * P.n3 + sid] * r_r);
+    put_prim(prim + (long long)(NUM_STATE + 1 + tr) * P.nh + hid, (ST)((double)U[(long long)(NUM_STATE + tr) * P.n3 + sid] * r_r));
 }
 
 // ------------------------------------------------------------------------------------------------
