@@ -27,9 +27,9 @@ struct PrepParams {
 // ------------------------------------------------------------------------------------------------
 // prep: primitive fields with halos (cross-shaped: the halo corners are never read)
 // ------------------------------------------------------------------------------------------------
-// Stores of the primitive fields, non-temporal (PU_NT_STORE=0: ordinary stores): the fields are far
-// larger than the caches and the next kernel reads them from HBM either way.  Measured at
-// 1024x1024x128 WENO-9 FP64: the step 186.4 -> 185.2 ms.
+// Stores of the primitive fields and of the coupler state, non-temporal (PU_NT_STORE=0: ordinary
+// stores): the fields are far larger than the caches and the next kernel reads them from HBM
+// either way.  Measured at 1024x1024x128 WENO-9 FP64 (k_prep only): the step 186.4 -> 185.2 ms.
 #ifndef PU_NT_STORE
 #define PU_NT_STORE 1
 #endif
@@ -193,16 +193,16 @@ __global__ __launch_bounds__(256) void k_c2d_prep(const double *__restrict__ rd_
   double rt = stored_cons<ST>(idT, x[idT]);
   int kc = P.hs + ks;
   double r_r = 1. / rho;
-  prim[(long long)idP * P.nh + hid] = (ST)(P.C0 * pow(rt, P.gamma) - P.hy_p[kc]);
-  prim[(long long)idR * P.nh + hid] = (ST)(rho - P.hy_r[kc]);
-  prim[(long long)idU * P.nh + hid] = (ST)(stored_cons<ST>(idU, x[idU]) * r_r);
-  prim[(long long)idV * P.nh + hid] = (ST)(stored_cons<ST>(idV, x[idV]) * r_r);
-  prim[(long long)idW * P.nh + hid] = zwall ? (ST)0 : (ST)(stored_cons<ST>(idW, x[idW]) * r_r);
-  prim[(long long)idT * P.nh + hid] = zwall ? (ST)0 : (ST)(rt * r_r - P.hy_t[kc]);
+  put_prim(prim + (long long)idP * P.nh + hid, (ST)(P.C0 * pow(rt, P.gamma) - P.hy_p[kc]));
+  put_prim(prim + (long long)idR * P.nh + hid, (ST)(rho - P.hy_r[kc]));
+  put_prim(prim + (long long)idU * P.nh + hid, (ST)(stored_cons<ST>(idU, x[idU]) * r_r));
+  put_prim(prim + (long long)idV * P.nh + hid, (ST)(stored_cons<ST>(idV, x[idV]) * r_r));
+  put_prim(prim + (long long)idW * P.nh + hid, zwall ? (ST)0 : (ST)(stored_cons<ST>(idW, x[idW]) * r_r));
+  put_prim(prim + (long long)idT * P.nh + hid, zwall ? (ST)0 : (ST)(rt * r_r - P.hy_t[kc]));
 #pragma unroll
   for (int tr = 0; tr < PU_MAX_TRACERS; tr++)
     if (tr < P.ntr)
-      prim[(long long)(NUM_STATE + 1 + tr) * P.nh + hid] = (ST)(stored_cons<ST>(NUM_STATE + tr, x[NUM_STATE + tr]) * r_r);
+      put_prim(prim + (long long)(NUM_STATE + 1 + tr) * P.nh + hid, (ST)(stored_cons<ST>(NUM_STATE + tr, x[NUM_STATE + tr]) * r_r));
 }
 
 template <class ST>
@@ -227,10 +227,10 @@ __global__ __launch_bounds__(256) void k_d2c(const ST *__restrict__ U, const ST 
   for (int tr = 0; tr < PU_MAX_TRACERS; tr++)
     if (tr < P.ntr && ((P.adds_mass_mask >> tr) & 1u)) rho_d -= m[tr];
   double temp = press / (rho_d * P.R_d + rho_v * P.R_v);
-  rd_[id] = rho_d; uu[id] = u; vv[id] = v; ww[id] = w; tt[id] = temp;
+  put_prim(rd_ + id, rho_d); put_prim(uu + id, u); put_prim(vv + id, v); put_prim(ww + id, w); put_prim(tt + id, temp);
 #pragma unroll
   for (int tr = 0; tr < PU_MAX_TRACERS; tr++)
-    if (tr < P.ntr) trc.p[tr][id] = m[tr];
+    if (tr < P.ntr) put_prim(trc.p[tr] + id, m[tr]);
 }
 
 template <class ST>
