@@ -183,16 +183,25 @@ static double map_rs(double w, double d) {             /* WenoLimiter.h:76-82, k
   return d + (term1 * (w - d)) / (term1 + 2000. * term2);
 }
 
-static double gll_face(const weno_tab *t, const double *TV, const double *ideal,
-                       const double (*face)[MAXORD], int do_map, const double *s) {
+/* Final (normalised) weights of one face: w_m = d_m/(TV_m^2 + 1e-20), convexified, mapped by
+ * map_rs (map for ord 3) and convexified again when do_map (WenoLimiter.h gll_1/gll_2). */
+static void face_weights(const weno_tab *t, const double *TV, const double *ideal, int do_map, double *w) {
   int S = t->S;
-  double w[MAXS];
   for (int m = 0; m < S; m++) w[m] = ideal[m] / (TV[m] * TV[m] + 1.e-20);
   convexify(S, w);
   if (do_map) {
     for (int m = 0; m < S; m++) w[m] = t->ord == 3 ? map_w(w[m], ideal[m]) : map_rs(w[m], ideal[m]);
     convexify(S, w);
   }
+}
+
+double oc_weno_map_rs(double w, double d) { return map_rs(w, d); }
+
+static double gll_face(const weno_tab *t, const double *TV, const double *ideal,
+                       const double (*face)[MAXORD], int do_map, const double *s) {
+  int S = t->S;
+  double w[MAXS];
+  face_weights(t, TV, ideal, do_map, w);
   double q = 0;
   for (int m = 0; m < S; m++) {
     double v = 0;
@@ -202,10 +211,10 @@ static double gll_face(const weno_tab *t, const double *TV, const double *ideal,
   return q;
 }
 
-static void limited_edges(const weno_tab *t, const double *s, int do_map, int imm_L, int imm_R,
-                          double *qL, double *qR) {
+/* Smoothness indicators of the S sub-stencils (coefs_to_tv of sten_to_coefs), convexified, with
+ * the imm_L / imm_R replacement by the maximum (WenoLimiter.h compute_limited_edges). */
+static void stencil_tv(const weno_tab *t, const double *s, int imm_L, int imm_R, double *TV) {
   int S = t->S, N = t->N;
-  double TV[MAXS];
   for (int m = 0; m < S; m++) {
     double a[MAXS];
     for (int n = 0; n < N; n++) {
@@ -220,8 +229,25 @@ static void limited_edges(const weno_tab *t, const double *s, int do_map, int im
   convexify(S, TV);
   if (imm_L) { double mx = TV[0]; for (int m = 1; m < S; m++) mx = fmax(mx, TV[m]); TV[S - 1] = mx; }
   if (imm_R) { double mx = TV[0]; for (int m = 1; m < S; m++) mx = fmax(mx, TV[m]); TV[0] = mx; }
+}
+
+static void limited_edges(const weno_tab *t, const double *s, int do_map, int imm_L, int imm_R,
+                          double *qL, double *qR) {
+  double TV[MAXS];
+  stencil_tv(t, s, imm_L, imm_R, TV);
   *qL = gll_face(t, TV, t->idealL, (const double (*)[MAXORD])t->faceL, do_map, s);
   *qR = gll_face(t, TV, t->idealR, (const double (*)[MAXORD])t->faceR, do_map, s);
+}
+
+int oc_weno_face_weights(int ord, const double *stencil, int do_map, int imm_L, int imm_R,
+                         double *wL, double *wR) {
+  const weno_tab *t = get_tab(ord);
+  if (!t) return -1;
+  double TV[MAXS];
+  stencil_tv(t, stencil, imm_L, imm_R, TV);
+  face_weights(t, TV, t->idealL, do_map, wL);
+  face_weights(t, TV, t->idealR, do_map, wR);
+  return 0;
 }
 
 int oc_weno_limited_edges(int ord, const double *stencil, int do_map, int imm_L, int imm_R,

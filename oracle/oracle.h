@@ -64,6 +64,14 @@ int  oc_weno_tables(int ord, double *C, double *faceL, double *faceR, double *id
 int  oc_weno_limited_edges(int ord, const double *stencil, int do_map, int imm_L, int imm_R,
                            double *qL, double *qR);
 
+/* The final normalised weights of the two faces that compute_limited_edges combines the
+ * sub-stencil values with (wL, wR: [(ord+1)/2]), the same code path as oc_weno_limited_edges. */
+int  oc_weno_face_weights(int ord, const double *stencil, int do_map, int imm_L, int imm_R,
+                          double *wL, double *wR);
+
+/* The weight map of WenoLimiter.h:76-82 (map_rs, k = 6, m = 3, s = 2000) the limiter applies. */
+double oc_weno_map_rs(double w, double d);
+
 /* Dynamics_Euler_Stratified_WenoFV::compute_time_step (dynamics_rk.h:57-64). */
 double oc_dycore_compute_time_step(const oc_config *cfg);
 

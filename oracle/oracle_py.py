@@ -10,7 +10,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# PORTURB_ORACLE_LIB selects another build of the same sources (the planted-mistake checks of
+# tests/test_oracle_mutants.py load mutated copies this way)
+LIB_PATH = os.environ.get("PORTURB_ORACLE_LIB", os.path.join(HERE, "liboracle.so"))
 MAXT = 16
 
 
@@ -57,6 +59,10 @@ def lib():
         L.oc_weno_tables.restype = I
         L.oc_weno_limited_edges.argtypes = [I, P, I, I, I, P, P]
         L.oc_weno_limited_edges.restype = I
+        L.oc_weno_face_weights.argtypes = [I, P, I, I, I, P, P]
+        L.oc_weno_face_weights.restype = I
+        L.oc_weno_map_rs.argtypes = [D, D]
+        L.oc_weno_map_rs.restype = D
         L.oc_dycore_compute_time_step.argtypes = [cp]
         L.oc_dycore_compute_time_step.restype = D
         L.oc_dycore_create.argtypes = [cp, P, P]
@@ -153,6 +159,21 @@ def weno_edges(ord_, stencil, do_map=True, imm_L=False, imm_R=False):
                                      C.byref(qL), C.byref(qR))
     assert rc == 0
     return qL.value, qR.value
+
+
+def weno_face_weights(ord_, stencil, do_map=True, imm_L=False, imm_R=False):
+    """Final normalised weights (left face, right face) of the limiter on one stencil."""
+    s = np.ascontiguousarray(stencil, dtype=np.float64)
+    S = (ord_ + 1) // 2
+    wL, wR = np.zeros(S), np.zeros(S)
+    rc = lib().oc_weno_face_weights(ord_, s.ctypes.data, int(do_map), int(imm_L), int(imm_R),
+                                    wL.ctypes.data, wR.ctypes.data)
+    assert rc == 0
+    return wL, wR
+
+
+def weno_map_rs(w, d):
+    return lib().oc_weno_map_rs(float(w), float(d))
 
 
 class Dycore:
