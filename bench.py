@@ -127,7 +127,7 @@ def main():
     d = pa.Driver(cfg["case_id"], cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlen"] / cfg["nx"], ord=cfg["ord"],
                   precision=cfg["precision"], seed=1234, closure=les, sgs_model=SGS_MODEL if learned else None)
     if turbines:
-        d.set_turbines(TURBINE_FILE, 4, 3, orig_C_T=True)
+        d.set_turbines(TURBINE_FILE, 3, 4, orig_C_T=True)
 
     def step():
         d.step(1)

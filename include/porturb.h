@@ -267,7 +267,7 @@ int  pu_kernel_time(pu_ctx *ctx, const char *name, double *total_ms, long long *
 /* Benchmark driver on the drop-in C++ API (core::Coupler + modules), the analog of the reference's
  * experiments/examples/abl_neutral.cpp:65-179 main loop: constants as sc_init.h:31-51, tracers
  * water_vapor (+ TKE from LES_Closure::init when closure != 0, + passive_1/2 for the ABL cases),
- * pu_case_init, dycore init; pu_driver_step runs, with dt = compute_time_step, nsteps times:
+ * pu_case_init, dycore init; pu_driver_step runs, with dt = compute_time_step * dyn_cycle, nsteps times:
  * run_module(geostrophic_wind_forcing) and run_module(uniform_pg_wind_forcing[_height]) when
  * enabled, run_module(dycore.time_step), run_module(sponge_layer) when enabled,
  * run_module(apply_surface_fluxes) for the ABL cases and run_module(les_closure.apply): the module
@@ -302,6 +302,11 @@ int  pu_driver_restart(pu_driver *d, const char *path);
  * ncols_x x nrows_y turbines at x = (i+1) xlen/(ncols_x+1), y = (j+1/2) ylen/nrows_y, fixed yaw 0,
  * upstream direction 0, no blades; orig_C_T sets option turbine_orig_C_T.  Needs the closure. */
 int  pu_driver_set_turbines(pu_driver *d, const char *turbine_file, int ncols_x, int nrows_y, int orig_C_T);
+/* The physics step of pu_driver_step: dt = compute_time_step * dyn_cycle (default 1), as the
+ * reference's ABL drivers set it (abl_neutral.cpp:27,86, city.cpp:52,123: dyn_cycle = 10).  The
+ * dycore runs dyn_cycle SSPRK3 cycles per step (dynamics_rk.h:103-137), the forcing, surface
+ * fluxes, turbines and closure once with the whole step. */
+int  pu_driver_set_dyn_cycle(pu_driver *d, int dyn_cycle);
 pu_turbines *pu_driver_turbines(pu_driver *d);
 /* option elapsed_time of the driver's coupler (s) */
 double pu_driver_elapsed_time(pu_driver *d);

@@ -97,6 +97,7 @@ def lib():
         "pu_driver_restart": (I, [P, C.c_char_p]),
         "pu_driver_elapsed_time": (D, [P]),
         "pu_driver_set_turbines": (I, [P, C.c_char_p, I, I, I]),
+        "pu_driver_set_dyn_cycle": (I, [P, I]),
         "pu_driver_turbines": (P, [P]),
         "pu_driver_state": (I, [P, csp, C.POINTER(I)]),
         "pu_driver_context": (P, [P]),
@@ -729,6 +730,11 @@ class Driver:
         check(lib().pu_turbines_status(lib().pu_driver_turbines(self.d), i, C.byref(st)))
         return st
 
+    def set_dyn_cycle(self, n):
+        """Physics step = n dycore steps (the reference ABL drivers' dyn_cycle)."""
+        if lib().pu_driver_set_dyn_cycle(self.d, int(n)) != PU_OK:
+            raise ValueError("pu_driver_set_dyn_cycle: " + lib().pu_driver_last_error().decode())
+
     def step(self, n=1):
         if lib().pu_driver_step(self.d, n) != PU_OK:
             raise RuntimeError("pu_driver_step: " + lib().pu_driver_last_error().decode())
@@ -758,6 +764,37 @@ class Driver:
             check(lib().pu_memcpy_d2h(a.ctypes.data, cs.tracers[t], nb))
             out[name] = a
         return out
+
+    def field_pointers(self):
+        """{name: device pointer} of density_dry, uvel, vvel, wvel, temp and every tracer."""
+        cs, n = PuCouplerState(), C.c_int()
+        check(lib().pu_driver_state(self.d, C.byref(cs), C.byref(n)))
+        out = {f: getattr(cs, f) for f in CouplerState.FIELDS}
+        for t, name in enumerate(self.tracer_names()):
+            out[name] = cs.tracers[t]
+        return out
+
+    def download_field(self, name):
+        a = np.empty(self.shape)
+        check(lib().pu_memcpy_d2h(a.ctypes.data, self.field_pointers()[name], a.nbytes))
+        return a
+
+    def roll_y(self, shift):
+        """Periodic shift of every coupler field by `shift` cells along y, on the device
+        (np.roll(field, shift, axis=1) for [nz][ny][nx])."""
+        nz, ny, nx = self.shape
+        shift %= ny
+        row = nx * 8
+        tmp = DeviceBuffer(nz * ny * row)
+        for ptr in self.field_pointers().values():
+            for k in range(nz):
+                src, dst = ptr + k * ny * row, tmp.ptr + k * ny * row
+                check(lib().pu_memcpy_d2d(C.c_void_p(dst + shift * row), C.c_void_p(src), (ny - shift) * row))
+                if shift:
+                    check(lib().pu_memcpy_d2d(C.c_void_p(dst), C.c_void_p(src + (ny - shift) * row), shift * row))
+            check(lib().pu_memcpy_d2d(C.c_void_p(ptr), C.c_void_p(tmp.ptr), nz * ny * row))
+        check(lib().pu_synchronize(self.ctx))
+        tmp.free()
 
     def synchronize(self):
         check(lib().pu_synchronize(self.ctx))

@@ -3,7 +3,7 @@
 // Mirrors the structure of the reference's ABL drivers (experiments/examples/abl_neutral.cpp:
 // 65-179): set options and constants as custom_modules::sc_init does (sc_init.h:31-51), allocate
 // the coupled state, add tracers, initialise the case, init the LES closure and the dycore, then
-// per step, with dt = dycore.compute_time_step: the optional column forcing
+// per step, with dt = dycore.compute_time_step * dyn_cycle: the optional column forcing
 // (geostrophic_wind_forcing, uniform_pg_wind_forcing[_height]), run_module(dycore.time_step), the
 // optional sponge_layer, run_module(apply_surface_fluxes) for the rough-wall ABL cases, the
 // optional actuator-disk turbines and run_module(les_closure.apply) -- the reference's module
@@ -33,6 +33,7 @@ struct pu_driver {
   pu_driver_forcing forcing{};
   modules::WindmillActuators windmills;
   bool turbines = false;
+  int dyn_cycle = 1;        // dycore SSPRK3 cycles per physics step (abl_neutral.cpp:27,86)
   std::string error;
 };
 
@@ -144,6 +145,15 @@ int pu_driver_set_turbines(pu_driver *d, const char *turbine_file, int ncols_x, 
   }
 }
 
+int pu_driver_set_dyn_cycle(pu_driver *d, int dyn_cycle) {
+  if (!d || dyn_cycle < 1) {
+    g_driver_error = "pu_driver_set_dyn_cycle: dyn_cycle must be >= 1";
+    return PU_ERR_ARG;
+  }
+  d->dyn_cycle = dyn_cycle;
+  return PU_OK;
+}
+
 pu_turbines *pu_driver_turbines(pu_driver *d) { return d && d->turbines ? d->windmills.handle() : nullptr; }
 
 int pu_driver_step(pu_driver *d, int nsteps) {
@@ -152,7 +162,9 @@ int pu_driver_step(pu_driver *d, int nsteps) {
     core::Coupler &c = d->coupler;
     const pu_driver_forcing &f = d->forcing;
     for (int n = 0; n < nsteps; n++) {
-      double dt = d->dycore.compute_time_step(c);
+      // the physics step: dyn_cycle dycore steps (abl_neutral.cpp:86); the dycore splits it into
+      // dyn_cycle SSPRK3 cycles, every other module runs once with the whole step
+      double dt = d->dycore.compute_time_step(c) * d->dyn_cycle;
       if (f.geostrophic)
         c.run_module([&](core::Coupler &cc) { modules::geostrophic_wind_forcing(cc, dt, f.lat_g, f.u_g, f.v_g); },
                      "geostrophic_forcing");

@@ -10,8 +10,9 @@
 //      yawed and binned into cells (round(x/dx - 0.5)), weighted by the radial thrust shape times
 //      the axial projection shape, once at the disk and once 4 R upstream (:654-716).  One thread
 //      walks one axial line of the cloud and adds runs of points that land in the same cell with
-//      one atomic; the weights are 2^-40 fixed point in 64-bit integers, so the sums are exact and
-//      independent of the order of the atomics (the reference adds floats atomically);
+//      one atomic; the weights are 2^-b fixed point in 64-bit integers (b <= 40, chosen so that
+//      no sum can exceed 2^62: TURB_FIX_MAX_BITS), so the sums are exact and independent of the
+//      order of the atomics (the reference adds floats atomically);
 //   3. the weight totals (exact integer sums), the normalised weights, and the sampled inflow
 //      sum(w_samp u), sum(w_samp v) in a fixed reduction order (:834-897);
 //   4. on the host: C_T, C_P, power from the table at the inertially filtered inflow, the inertial
@@ -29,7 +30,11 @@
 
 namespace pu {
 
-constexpr double TURB_FIX = 1099511627776.0;     // 2^40: fixed-point scale of the cloud weights
+// Fixed-point scale of the cloud weights: 2^fix_bits with fix_bits = min(40, 62 - ceil(log2(points))),
+// points = num_x num_y num_z.  Every point's weight is at most 1 (thrust_shape2 and proj_shape_1d
+// peak at 1), so a disk's total, and every cell's partial sum, stays below 2^62 in the unsigned
+// 64-bit sums.  (A rotor of 60 m on 5 m cells has 7.4e7 points: at 2^40 its total would wrap.)
+constexpr int TURB_FIX_MAX_BITS = 40;
 
 struct TurbBox {
   int i0, j0, k0, ni, nj, nk;                     // cell-index origin and extents (clipped to the domain)
@@ -40,6 +45,7 @@ struct CloudArgs {
   int nx, ny, nz, num_x, num_y, num_z;
   double dx, dy, dz, xr, R, rad, hub_radius, decayloc, overhang;
   double cos_yaw, sin_yaw, cos_tlt, sin_tlt, base_x, base_y, hub_height, up_x, up_y;
+  double fix;                                     // fixed-point scale (2^fix_bits)
   TurbBox pbox, sbox;
   unsigned long long *acc_p, *acc_s;
   unsigned int *err;
@@ -87,7 +93,7 @@ __global__ __launch_bounds__(256) void k_turbine_cloud(CloudArgs A) {
   for (int i = 0; i < A.num_x; i++) {
     const double x = -A.xr + (2 * A.xr * i) / (A.num_x - 1);
     const double shp = radial * proj_shape_1d(x, A.xr);
-    const unsigned long long q = (unsigned long long)llrint(shp * TURB_FIX);
+    const unsigned long long q = (unsigned long long)llrint(shp * A.fix);
     const double x1 = A.cos_tlt * (x + A.overhang) + A.sin_tlt * z;
     const double y1 = y;
     const double z1 = -A.sin_tlt * (x + A.overhang) + A.cos_tlt * z;
@@ -263,6 +269,7 @@ struct pu_turbines {
   long long box_cap = 0;
   int num_x = 0, num_y = 0, num_z = 0;
   double xr = 0, R = 0, decayloc = 0, Rb = 0;
+  double fix = 0;                           // fixed-point scale of the cloud weights
   unsigned long long *d_acc = nullptr;      // [2][box_cap]
   double *d_tend = nullptr;                 // [nturb][3][box_cap]
   double *d_part = nullptr;                 // [2 * TURB_RED_BLOCKS]
@@ -335,6 +342,12 @@ int pu_turbines_create(pu_ctx *ctx, const pu_turbine_ref *ref, int nturb, const 
   T->num_x = (int)std::ceil(20 / dx * T->xr * 2);
   T->num_y = (int)std::ceil(20 / dx * T->rad * (1 + T->decayloc) * 2);
   T->num_z = T->num_y;
+  {
+    const double points = (double)T->num_x * T->num_y * T->num_z;
+    const int bits = std::min(TURB_FIX_MAX_BITS, 62 - (int)std::ceil(std::log2(points)));
+    if (bits < 16) { delete T; return fail(PU_ERR_ARG, "pu_turbines_create: rotor cloud too fine for the fixed-point sums"); }
+    T->fix = std::ldexp(1.0, bits);
+  }
   T->Rb = std::sqrt((T->xr + std::fabs(T->overhang)) * (T->xr + std::fabs(T->overhang)) + T->R * T->R);
   const double dy = c.ylen / c.ny, dz = c.zlen / c.nz;
   T->box_cap = (long long)(2 * (int)std::ceil(T->Rb / dx) + 5) * (2 * (int)std::ceil(T->Rb / dy) + 5) *
@@ -422,6 +435,7 @@ int pu_turbines_apply(pu_turbines *T, pu_coupler_state *cs, double dt, double *p
     A.decayloc = T->decayloc; A.overhang = T->overhang;
     A.cos_yaw = cos_yaw; A.sin_yaw = sin_yaw; A.cos_tlt = cos_tlt; A.sin_tlt = sin_tlt;
     A.base_x = hx; A.base_y = hy; A.hub_height = T->hub_height; A.up_x = up_x; A.up_y = up_y;
+    A.fix = T->fix;
     A.pbox = pbox; A.sbox = sbox; A.acc_p = acc_p; A.acc_s = acc_s; A.err = T->d_err;
     const long long lines = (long long)T->num_y * T->num_z;
     hipLaunchKernelGGL(k_turbine_cloud, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, s, A);

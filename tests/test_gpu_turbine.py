@@ -143,3 +143,25 @@ def test_driver_windfarm_turbines_match_raw_calls():
     assert T.status(0).ct == 0.75 and T.status(0).u_samp == st.u_samp
     T.destroy()
     drv.destroy()
+
+
+def test_full_size_rotor_sums_do_not_overflow():
+    """A 120 m rotor (R = 60 m) on 5 m cells, the wind-farm workload's turbine: its point cloud has
+    200 x 610 x 610 = 7.4e7 points, whose fixed-point total must stay exact (at 2^40 per point it
+    would pass 2^64).  The normalised projection and sampling weights sum to one, and the whole step
+    matches the oracle."""
+    cfg = small_config(nx=100, ny=40, nz=40, tracers=[WV, TKE], dx=5.0)
+    c = flow(cfg, 13, u0=9.0)
+    tab = ([0.0, 100.0], [0.75, 0.75], [0.0, 0.0], [0.0, 0.0])
+    kw = dict(initial_yaw=0.0, fixed_yaw=True, orig_C_T=True, upstream_dir=0.0)
+    okw = dict(fixed_yaw=True, orig_C_T=True, upstream_dir=0.0)
+    print("rotor R = 60 m, dx = 5 m", flush=True)
+    T, tu, g, oc, gp, gs, pw, sw = run_both(cfg, c, tab, kw, okw, 2, 0.5, [400.0], [100.0], hub_height=90.0,
+                                            blade_radius=60.0)
+    assert gp.sum() == pytest.approx(1.0, rel=1e-12) and gs.sum() == pytest.approx(1.0, rel=1e-12)
+    assert pw.sum() == pytest.approx(1.0, rel=1e-12)
+    assert close(gp, pw, 1e-10) and close(gs, sw, 1e-10)
+    for f in (1, 6):
+        assert close(g[f] - c[f], oc[f] - c[f], 1e-9), f
+    assert T.status(0).u_samp == pytest.approx(tu[0].u_samp, rel=1e-10)
+    T.destroy()
