@@ -48,7 +48,6 @@ static int validate(const pu_config *c) {
   if (c->bc_x == PU_BC_PERIODIC && c->nx < hs + 1) return fail(PU_ERR_UNSUPPORTED, "nx smaller than the stencil halo");
   if (c->bc_y == PU_BC_PERIODIC && c->ny < hs + 1 && c->ny > 1) return fail(PU_ERR_UNSUPPORTED, "ny smaller than the stencil halo");
   if (c->nz < hs + 1) return fail(PU_ERR_UNSUPPORTED, "nz smaller than the stencil halo");
-  if (c->ny == 1) return fail(PU_ERR_UNSUPPORTED, "2-D (ny == 1) runs are not supported by the GPU path");
   return PU_OK;
 }
 

@@ -243,6 +243,7 @@ __global__ __launch_bounds__(256) void k_tend_export(const ST *__restrict__ tend
   double rho_in = load_cons(U, Ulo, idR, id, P.n3);
   for (int l = 0; l < nv; l++) {
     double tn = (double)tend[(long long)l * P.n3 + id];
+    if (l == idV && P.sim2d) tn = 0.;
     if (l == idW && P.enable_gravity) tn += -P.grav * (rho_in - P.hy_r[P.hs + k]);
     if (P.coriolis && l == idU) tn += P.fcor * load_cons(U, Ulo, idV, id, P.n3);
     if (P.coriolis && l == idV) tn -= P.fcor * load_cons(U, Ulo, idU, id, P.n3);
@@ -385,7 +386,10 @@ static CombineParams combine_params(pu_ctx *ctx) {
   P.n3 = ctx->g.n3();
   P.enable_gravity = ctx->cfg.enable_gravity;
   P.grav = ctx->cfg.grav;
-  P.coriolis = ctx->cfg.latitude != 0 && ctx->g.ny > 1;
+  // a 2-D run (ny == 1, Coupler::is_sim2d) has no y fluxes, no Coriolis and a zero V tendency
+  // (dynamics_rk.h:663,716,720-721); the y sweep is not launched
+  P.sim2d = ctx->g.ny == 1;
+  P.coriolis = ctx->cfg.latitude != 0 && !P.sim2d;
   P.fcor = 2 * 7.2921e-5 * sin(ctx->cfg.latitude / 180 * M_PI);
   P.hy_r = hy_cells(ctx, 0); P.hy_t = hy_cells(ctx, 1);
   P.relax = ctx->d_relax;

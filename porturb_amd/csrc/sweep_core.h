@@ -429,6 +429,7 @@ __global__ __launch_bounds__(ZW_BX * ZW_BS) void k_sweep_zwall(const T *__restri
 
 struct CombineParams {
   int nx, ny, nz, hs, ntr, stage, nrelax, enable_gravity, coriolis;
+  int sim2d;                        // ny == 1: the V tendency is zero (dynamics_rk.h:716)
   long long n3;
   double dt, grav, fcor;
   const double *hy_r, *hy_t;        // cells
@@ -477,6 +478,7 @@ __device__ __forceinline__ void combine_field_v(const FuseArgs<ST> &F, int l, lo
                                                 double us, double rho_in, double u_in, double v_in, double mult,
                                                 double tr_r, double tr_rt) {
   const CombineParams &P = F.C;
+  if (l == idV && P.sim2d) tn = 0.;
   if (l == idW && P.enable_gravity) tn += -P.grav * (rho_in - tr_r);
   if (P.coriolis && l == idU) tn += P.fcor * v_in;
   if (P.coriolis && l == idV) tn -= P.fcor * u_in;

@@ -173,3 +173,24 @@ def test_les_momentum_flux_is_down_gradient_and_feeds_shear_production(S):
     cfg, st, imm = pc.les_shear(S)
     cfg0, st0, _ = pc.les_shear(0.)
     pc.check_les_shear(cfg, st, _les(cfg, st, imm), _les(cfg0, st0, imm), S, LES_DT)
+
+
+def test_2d_runs_have_no_v_tendency_and_no_coriolis():
+    """ny == 1 (dynamics_rk.h:663,716,720-721): V's tendency is zero and the latitude changes
+    nothing; the same state in 3-D (ny = 4 copies) has both."""
+    from tests.helpers import small_config, random_state, WV, P1
+    cfg = small_config(nx=30, ny=1, nz=12, ord=5, tracers=[WV, P1], latitude=35.)
+    st, imm = random_state(cfg, seed=7)
+    t_lat = oracle_py.Dycore(cfg, st, imm).tendencies(st)
+    t_eq = oracle_py.Dycore(dict(cfg, latitude=0.), st, imm).tendencies(st)
+    assert np.array_equal(t_lat[2], np.zeros_like(t_lat[2]))
+    assert np.array_equal(t_lat, t_eq)
+    cfg3 = small_config(nx=30, ny=4, nz=12, ord=5, tracers=[WV, P1], latitude=35.)
+    st3 = np.repeat(st, 4, axis=2)
+    t3 = oracle_py.Dycore(cfg3, st3, np.zeros(st3.shape[1:])).tendencies(st3)
+    assert np.abs(t3[2]).max() > 0
+    # x and z fluxes are the same in 2-D and in the y-uniform 3-D state: only V and the Coriolis
+    # term of U differ
+    t3_eq = oracle_py.Dycore(dict(cfg3, latitude=0.), st3, np.zeros(st3.shape[1:])).tendencies(st3)
+    np.testing.assert_allclose(t3_eq[[0, 3, 4, 5, 6], :, :1], t_eq[[0, 3, 4, 5, 6]], rtol=1e-9,
+                               atol=1e-12 * np.abs(t_eq).max())
