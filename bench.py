@@ -8,8 +8,17 @@ closure, FP64.  One step = one Dynamics_Euler_Stratified_WenoFV::time_step (3 st
 apply_surface_fluxes + one LES_Closure::apply, inputs resident in HBM.  Timed with HIP events on
 the library's stream.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
-                    [--config abl_fp64|abl_fp32|abl_fp32_learned|bubble|windfarm]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 200]
+                    [--config abl_fp64|abl_fp32|abl_fp32_learned|bubble|windfarm] [--no-modes]
+
+The timed steps start from a spun-up state: the seeded initialisation advanced by --warmup steps.
+The dominant sweep's roofline counts the FLOPs the sweep executes on that state: a census of every
+reconstruction's branch (the full limiter, the flat-stencil branch that takes the linear
+reconstruction where the stencil's total variation is <= 1e-20, or the linear reconstruction of a
+cell without WENO) at the start and the end of the timed steps; the full-limiter figure is kept
+beside it.  With the default workload (abl_fp64) the same spun-up state is then advanced by the
+FP32 dycore with the TKE closure and with the learned closure: their ms/step, speed-up over FP64
+and per-field error against the FP64 state after the same steps are in "modes".
 
 abl_fp32_learned: the FP32 dycore with the learned subgrid closure (network file
 porturb_amd/data/sgs_abl_urban_dx2.bin, trained by porturb_amd.sgs_training) in place of the
@@ -47,16 +56,16 @@ def make_workload(name):
 
 def cpu_baseline(cfg_full, les, seconds_budget=25.0):
     """The CPU oracle (FP64 restatement of the reference, OpenMP over all host cores) on a bounded
-    sample of the same workload: the same case, order, tracers and closure on a 128x128xnz box
-    (bubble: the full 64^3 case), one step."""
+    sample of the same workload: the same case, order, tracers and closure on a 256x256xnz box
+    (bubble: the full 64^3 case), one or two steps from the seeded initialisation."""
     import numpy as np
     import porturb_amd as pa
     from oracle import oracle_py
     cores = len(os.sched_getaffinity(0))
     cfg = dict(cfg_full)
-    if cfg["nx"] > 128:
+    if cfg["nx"] > 256:
         dx = cfg["xlen"] / cfg["nx"]
-        cfg.update(nx=128, ny=128, xlen=128 * dx, ylen=128 * dx)
+        cfg.update(nx=256, ny=256, xlen=256 * dx, ylen=256 * dx)
     cfg["precision"] = pa.PU_FP64
     cs = pa.CouplerState(cfg)
     pa.case_init(cfg, cfg["case_id"], 1234, cs)
@@ -77,7 +86,7 @@ def cpu_baseline(cfg_full, les, seconds_budget=25.0):
             oracle_py.les_apply(cfg, st, imm, dt, nthreads=cores)
         t_tot += time.perf_counter() - t0
         steps += 1
-        if t_tot > seconds_budget / 3 or steps >= 3:
+        if t_tot > seconds_budget / 2 or steps >= 2:
             break
     return {"value": n3 * steps / t_tot, "unit": "cell-updates/s", "cores": cores, "kind": "port",
             "sample": "%s on %dx%dx%d cells, %d step(s), %.1f s, FP64 OpenMP oracle" %
@@ -98,13 +107,83 @@ def sweep_kernel_name(cfg, fam):
     return "k_sweep<%d, %s, %d, %d>" % (cfg["ord"], t, d, {0: 2, 1: 1, 2: 0}[d])
 
 
+def snapshot(d):
+    """Device copies of every coupler field of driver `d`: {name: DeviceBuffer}."""
+    import porturb_amd as pa
+    nb = int(d.shape[0] * d.shape[1] * d.shape[2]) * 8
+    out = {}
+    for name, ptr in d.field_pointers().items():
+        b = pa.DeviceBuffer(nb)
+        pa.check(pa.lib().pu_memcpy_d2d(pa.C.c_void_p(b.ptr), pa.C.c_void_p(ptr), nb))
+        out[name] = b
+    d.synchronize()
+    return out
+
+
+def restore(d, snap):
+    import porturb_amd as pa
+    nb = int(d.shape[0] * d.shape[1] * d.shape[2]) * 8
+    for name, ptr in d.field_pointers().items():
+        pa.check(pa.lib().pu_memcpy_d2d(pa.C.c_void_p(ptr), pa.C.c_void_p(snap[name].ptr), nb))
+    d.synchronize()
+
+
+def compare_with(d, ref, shape):
+    """Per-field relative L2 / Linf of driver `d`'s state against the snapshot `ref`, and the
+    largest relative differences of the horizontal-mean wind and resolved-TKE profiles (the
+    profiles of build/postproc/abl_neutral_analysis.py:122-131)."""
+    import numpy as np
+    from porturb_amd.precision_study import profiles
+    fields, vel = {}, {}
+    for name, b in ref.items():
+        a = b.download(np.float64, shape)
+        x = d.download_field(name)
+        dd = x - a
+        fields[name] = {"l2": float(np.linalg.norm(dd) / max(np.linalg.norm(a), 1e-300)),
+                        "linf": float(np.abs(dd).max() / max(np.abs(a).max(), 1e-300))}
+        if name in ("uvel", "vvel", "wvel"):
+            vel[name] = (a, x)
+        del a, x, dd
+    p_ref = profiles(*(vel[n][0] for n in ("uvel", "vvel", "wvel")))
+    p_got = profiles(*(vel[n][1] for n in ("uvel", "vvel", "wvel")))
+    prof = {k: float(np.abs(p_got[k] - p_ref[k]).max() / max(np.abs(p_ref[k]).max(), 1e-300)) for k in p_ref}
+    return fields, prof
+
+
+def time_steps(d, steps):
+    d.synchronize()
+    d.timer_start()
+    for _ in range(steps):
+        d.step(1)
+    ms = d.timer_stop()
+    d.synchronize()
+    return ms / steps
+
+
+FAMS = ["weno_x", "weno_y", "weno_z", "prep", "surface", "les", "sgs", "turbines"]
+
+
+def kernel_times(d, steps=2):
+    """Per-launch HIP events around every kernel family for `steps` steps: {family: (ms, launches)}."""
+    d.kernel_timing(True)
+    for _ in range(steps):
+        d.step(1)
+    d.synchronize()
+    times = {f: d.kernel_time(f) for f in FAMS}
+    d.kernel_timing(False)
+    return times
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=200,
+                    help="steps from the seeded initialisation before the timed steps (spin-up)")
     ap.add_argument("--config", default="abl_fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="abl_fp64: skip the FP32 and FP32 + learned closure runs from the same state")
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -122,43 +201,44 @@ def main():
     # the drop-in C++ path: core::Coupler + Dynamics_Euler_Stratified_WenoFV (+ LES_Closure)
     learned = args.config in ("abl_fp32_learned", "windfarm")
     turbines = args.config == "windfarm"
-    if learned and not os.path.exists(SGS_MODEL):
+    modes = args.config == "abl_fp64" and not args.no_modes
+    if (learned or modes) and not os.path.exists(SGS_MODEL):
         raise SystemExit("bench: %s missing (train it with python -m porturb_amd.sgs_training)" % SGS_MODEL)
     d = pa.Driver(cfg["case_id"], cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlen"] / cfg["nx"], ord=cfg["ord"],
                   precision=cfg["precision"], seed=1234, closure=les, sgs_model=SGS_MODEL if learned else None)
     if turbines:
         d.set_turbines(TURBINE_FILE, 3, 4, orig_C_T=True)
 
-    def step():
+    t0 = time.time()
+    for w in range(args.warmup):
         d.step(1)
-
-    for _ in range(args.warmup):
-        step()
+        if (w + 1) % 25 == 0:
+            d.synchronize()
+            print("bench: warm-up step %d/%d (%.0f s)" % (w + 1, args.warmup, time.time() - t0),
+                  file=sys.stderr, flush=True)
     d.synchronize()
-    d.timer_start()
-    for _ in range(args.steps):
-        step()
-    ms = d.timer_stop()
-    d.synchronize()
-    ms_per_step = ms / args.steps
+    sim_time = d.elapsed_time()
+    # reconstruction census at the start and the end of the timed steps: the work they execute
+    census0 = d.flat_census()
+    snap0 = snapshot(d) if modes else None
+    ms_per_step = time_steps(d, args.steps)
     value = n3 / (ms_per_step * 1e-3)
+    census1 = d.flat_census()
+    snap1 = snapshot(d) if modes else None
     print("bench: %.3f ms/step, %.4e cell-updates/s" % (ms_per_step, value), file=sys.stderr, flush=True)
 
-    # per-kernel timing pass (HIP events around each launch on the library stream)
-    d.kernel_timing(True)
-    for _ in range(2):
-        step()
-    d.synchronize()
-    fams = ["weno_x", "weno_y", "weno_z", "prep", "surface", "les", "sgs", "turbines"]
-    times = {f: d.kernel_time(f) for f in fams}
-    d.kernel_timing(False)
-    dom = max(["weno_x", "weno_y", "weno_z"], key=lambda f: times[f][0])
+    times = kernel_times(d)
+    dom = max(["weno_x", "weno_y", "weno_z"], key=lambda f: times[f][0] / max(times[f][1], 1))
     tot_ms, cnt = times[dom]
     avg_ms = tot_ms / max(cnt, 1)
     nf = 6 + cfg["num_tracers"]
     fp64 = cfg["precision"] == pa.PU_FP64
-    flops_launch = n3 * rl.sweep_flops_per_cell(cfg["ord"], nf)
-    achieved = flops_launch / (avg_ms * 1e-3) / 1e12
+    drow = {"weno_x": 0, "weno_y": 1, "weno_z": 2}[dom]
+    census = (census0 + census1) / 2.0
+    flops_full = n3 * rl.sweep_flops_per_cell(cfg["ord"], nf)
+    flops_exec = rl.executed_sweep_flops(cfg["ord"], nf, census[drow], n3)
+    achieved = flops_exec / (avg_ms * 1e-3) / 1e12
+    achieved_full = flops_full / (avg_ms * 1e-3) / 1e12
     peak = rl.MI355X["fp64_TFLOPs"] if fp64 else rl.MI355X["fp32_TFLOPs"]
     step_bytes = rl.step_bytes_per_cell(5 + cfg["num_tracers"], fp64)
     hbm_frac = value * step_bytes / (rl.MI355X["hbm_GBps"] * 1e9)
@@ -171,12 +251,19 @@ def main():
         fam = json.load(open(tfile)).get("families", {}).get(dom)
         if fam:
             traffic = fam["read"] + fam["write"]
+    recons = census.sum(axis=1)
     roof = {"bound": "valu", "kernel": sweep_kernel_name(cfg, dom),
             "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic,
-            "flops_per_launch": flops_launch, "avg_launch_ms": round(avg_ms, 4),
+            "work": "executed: limiter, flat-stencil and linear reconstructions counted on the timed state",
+            "flops_per_launch": flops_exec, "avg_launch_ms": round(avg_ms, 4),
+            "achieved_full_limiter": round(achieved_full, 3), "frac_full_limiter": round(achieved_full / peak, 4),
+            "flops_per_launch_full_limiter": flops_full,
+            "flat_fraction": {k: round(float(census[i][1] / max(recons[i], 1)), 4) for i, k in enumerate("xyz")},
+            "census": {"x": census[0].tolist(), "y": census[1].tolist(), "z": census[2].tolist(),
+                       "columns": ["limiter", "flat", "linear"]},
             "step_hbm_frac": round(hbm_frac, 4), "step_bytes_per_cell": step_bytes,
-            "kernel_ms_per_step": {f: round(times[f][0] / 2, 3) for f in fams}}
+            "kernel_ms_per_step": {f: round(times[f][0] / 2, 3) for f in FAMS}}
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64" if fp64 else "f32", "data": "synthetic",
@@ -185,14 +272,36 @@ def main():
                       "path": "C++ Coupler + modules",
                       "closure": ("learned (bf16 matrix cores) + tke fallback" if learned else "tke") if les else "none",
                       "tracers": cfg["num_tracers"],
-                      # the timed steps follow the warm-up steps from the seeded case initialisation:
-                      # the air above the boundary layer is still uniform, where the limiters take
-                      # their flat-stencil branch (the linear reconstruction, equal to rounding)
-                      "state": "case initialisation + %d warm-up steps" % args.warmup,
+                      "state": "case initialisation + %d warm-up steps (%.3f s of simulated flow)" % (args.warmup, sim_time),
                       "step": ("dycore.time_step + apply_surface_fluxes + " +
                                ("windmills.apply + " if turbines else "") + "les_closure.apply") if les else "dycore.time_step"},
            "roofline": roof}
     d.destroy()
+
+    if modes:
+        # the same spun-up state advanced by the FP32 dycore (compensated rho, rho*theta) with the
+        # TKE closure and with the learned closure: speed against the FP64 run above and the
+        # per-field error after the same steps
+        shape = d.shape
+        out["modes"] = {}
+        for name, model in (("fp32", None), ("fp32_learned", SGS_MODEL)):
+            d2 = pa.Driver(cfg["case_id"], cfg["nx"], cfg["ny"], cfg["nz"], cfg["xlen"] / cfg["nx"], ord=cfg["ord"],
+                           precision=pa.PU_FP32, seed=1234, closure=les, sgs_model=model)
+            d2.step(2)                       # first launches of every kernel, outside the timing
+            restore(d2, snap0)
+            ms2 = time_steps(d2, args.steps)
+            fields, prof = compare_with(d2, snap1, shape)
+            t2 = kernel_times(d2)
+            out["modes"][name] = {
+                "ms_per_step": ms2, "value": n3 / (ms2 * 1e-3), "speedup_vs_fp64": ms_per_step / ms2,
+                "closure": "learned (bf16 matrix cores) + tke fallback" if model else "tke",
+                "err_vs_fp64_after_steps": args.steps, "err_vs_fp64": fields, "profile_rel_diff_vs_fp64": prof,
+                "kernel_ms_per_step": {f: round(t2[f][0] / 2, 3) for f in FAMS}}
+            print("bench: %s %.3f ms/step (%.3fx FP64)" % (name, ms2, ms_per_step / ms2), file=sys.stderr, flush=True)
+            d2.destroy()
+        for b in list(snap0.values()) + list(snap1.values()):
+            b.free()
+
     if not args.no_cpu_baseline:
         print("bench: CPU oracle baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(cfg, les, args.cpu_seconds)

@@ -97,6 +97,14 @@ int  pu_dycore_time_step(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys);
  * tend: device [5+num_tracers][nz][ny][nx] double.  For parity checking of a single stage. */
 int  pu_dycore_tendencies(pu_ctx *ctx, const pu_coupler_state *cs, double *tend);
 
+/* Reconstruction census of the first stage of a time step from `cs` (no counterpart in the
+ * reference: a measurement aid for the work the sweeps execute).  For each direction x, y, z and
+ * every interior cell and reconstructed field, the branch the WENO limiter takes on this state:
+ * counts[3*dir + 0] the full limiter, + 1 the flat-stencil branch (the candidate stencils' total
+ * variation <= 1e-20, where the limited edges equal the linear reconstruction to rounding), + 2
+ * the linear reconstruction of a cell without WENO.  Host output, synchronous. */
+int  pu_dycore_flat_census(pu_ctx *ctx, const pu_coupler_state *cs, long long *counts);
+
 /* Hydrostatic profiles as the dycore holds them (host outputs): hy_{dens,theta,pressure}_cells
  * [nz+2hs] and hy_{dens,theta,pressure}_edges [nz+1] (dynamics_rk.h:961-1008,1145-1186). */
 int  pu_dycore_hydrostatics(pu_ctx *ctx, double *dens_c, double *theta_c, double *pres_c,

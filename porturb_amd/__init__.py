@@ -72,6 +72,7 @@ def lib():
         "pu_dycore_compute_time_step": (D, [P]),
         "pu_dycore_time_step": (I, [P, csp, D]),
         "pu_dycore_tendencies": (I, [P, csp, P]),
+        "pu_dycore_flat_census": (I, [P, csp, P]),
         "pu_dycore_hydrostatics": (I, [P] + [P] * 6),
         "pu_dycore_immersed_masks": (I, [P, P, P]),
         "pu_les_closure_apply": (I, [P, csp, D]),
@@ -315,6 +316,13 @@ class Dycore:
         out = buf.download(np.float64, shape)
         buf.free()
         return out
+
+    def flat_census(self, cs):
+        """Reconstruction branches of the first stage of a time step from `cs`: int64 [3][3], rows
+        x, y, z, columns [limiter, flat stencil, linear]."""
+        out = np.zeros(9, dtype=np.int64)
+        check(lib().pu_dycore_flat_census(self.ctx, C.byref(cs.c), out.ctypes.data))
+        return out.reshape(3, 3)
 
     def hydrostatics(self):
         hs = (self.cfg["ord"] - 1) // 2
@@ -798,6 +806,15 @@ class Driver:
 
     def synchronize(self):
         check(lib().pu_synchronize(self.ctx))
+
+    def flat_census(self):
+        """Reconstruction branches of the first stage of the next step on the current state:
+        int64 [3][3], rows x, y, z, columns [limiter, flat stencil, linear]."""
+        cs, n = PuCouplerState(), C.c_int()
+        check(lib().pu_driver_state(self.d, C.byref(cs), C.byref(n)))
+        out = np.zeros(9, dtype=np.int64)
+        check(lib().pu_dycore_flat_census(self.ctx, C.byref(cs), out.ctypes.data))
+        return out.reshape(3, 3)
 
     def timer_start(self):
         check(lib().pu_timer_start(self.ctx))

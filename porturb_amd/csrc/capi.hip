@@ -213,6 +213,20 @@ int pu_dycore_tendencies(pu_ctx *ctx, const pu_coupler_state *cs, double *tend) 
   return PU_OK;
 }
 
+int pu_dycore_flat_census(pu_ctx *ctx, const pu_coupler_state *cs, long long *counts) {
+  int rc = check_state(ctx, cs, false);
+  if (rc) return rc;
+  if (!ctx->dycore_ready) return fail(PU_ERR_STATE, "pu_dycore_init has not been called");
+  if (!counts) return fail(PU_ERR_ARG, "null counts");
+  hipStream_t s = ctx->stream;
+  if ((rc = launch_c2d_prep(ctx, cs, s))) return rc;
+  PU_HIP_CHECK(hipMemsetAsync(ctx->d_red, 0, 9 * sizeof(unsigned long long), s));
+  if ((rc = launch_census(ctx, s, (unsigned long long *)ctx->d_red))) return rc;
+  PU_HIP_CHECK(hipMemcpyAsync(counts, ctx->d_red, 9 * sizeof(long long), hipMemcpyDeviceToHost, s));
+  PU_HIP_CHECK(hipStreamSynchronize(s));
+  return PU_OK;
+}
+
 int pu_dycore_hydrostatics(pu_ctx *ctx, double *dc, double *tc, double *pc, double *de, double *te, double *pe) {
   if (!ctx || !ctx->dycore_ready) return fail(PU_ERR_STATE, "dycore not initialised");
   double *outc[3] = {dc, tc, pc}, *oute[3] = {de, te, pe};

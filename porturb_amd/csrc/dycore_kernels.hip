@@ -252,6 +252,42 @@ __global__ __launch_bounds__(256) void k_tend_export(const ST *__restrict__ tend
 }
 
 // ------------------------------------------------------------------------------------------------
+// Reconstruction census of one sweep direction: for every interior cell and every field the sweep
+// reconstructs (rho, the normal velocity and p' first, then the transverse velocities, theta' and
+// the tracers, with the sweeps' zero-derivative clamps), the branch the limiter takes -- counts of
+// [limiter, flat stencil, linear] added to counts[0..2].  Measures the work the sweeps execute on a
+// given state.  grid: (blocks over a level's nx*ny cells, nz levels).
+// ------------------------------------------------------------------------------------------------
+template <int ORD, class T, int DIR>
+__global__ __launch_bounds__(256) void k_flat_census(const T *__restrict__ prim, const uint8_t *__restrict__ immf,
+                                                     const uint8_t *__restrict__ any10, SweepParams P,
+                                                     unsigned long long *counts) {
+  __shared__ unsigned long long sc[3];
+  if (threadIdx.x < 3) sc[threadIdx.x] = 0;
+  __syncthreads();
+  const int nxy = P.nx * P.ny;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  unsigned c[3] = {0u, 0u, 0u};
+  if (p < nxy) {
+    const int i = p % P.nx, j = p / P.nx;
+    const long long sstride = DIR == 0 ? 1 : (DIR == 1 ? P.nxh : P.nxh * P.nyh);
+    const long long base = ((long long)(k + P.hp) * P.nyh + (j + P.hp)) * P.nxh + (i + P.ox);
+    const long long cid = (long long)k * nxy + p;
+    StencilFlags<ORD> sf;
+    sf.setup(immf, base, sstride, any10[cid] != 0, P.weno_all);
+    constexpr int ln = 1 + DIR, lt1 = DIR == 0 ? idV : idU, lt2 = DIR == 2 ? idV : idW;
+    const int ls[5] = {idR, ln, idP, lt1, lt2};
+    for (int f = 0; f < 5; f++) c[sf.template branch<T>(prim + (long long)ls[f] * P.nh + base, sstride, f >= 2)]++;
+    c[sf.template branch<T>(prim + (long long)idT * P.nh + base, sstride, false)]++;
+    for (int tr = 0; tr < P.ntr; tr++)
+      c[sf.template branch<T>(prim + (long long)(NUM_STATE + 1 + tr) * P.nh + base, sstride, false)]++;
+  }
+  for (int q = 0; q < 3; q++) if (c[q]) atomicAdd(&sc[q], (unsigned long long)c[q]);
+  __syncthreads();
+  if (threadIdx.x < 3 && sc[threadIdx.x]) atomicAdd(&counts[threadIdx.x], sc[threadIdx.x]);
+}
+
+// ------------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------------
 static inline const double *hy_cells(pu_ctx *ctx, int w) { return ctx->d_hy + (size_t)w * (ctx->g.nz + 2 * ctx->g.hs); }
@@ -281,21 +317,27 @@ int launch_prep(pu_ctx *ctx, const void *U, const void *Ulo, hipStream_t s) {
 
 static CombineParams combine_params(pu_ctx *ctx);
 
+static SweepParams sweep_params(pu_ctx *ctx, int dir) {
+  const pu::Grid &g = ctx->g;
+  SweepParams P;
+  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ox = g.ox; P.ntr = ctx->cfg.num_tracers;
+  P.bc_along = dir == 0 ? ctx->cfg.bc_x : (dir == 1 ? ctx->cfg.bc_y : ctx->cfg.bc_z);
+  if (dir == 2 && P.bc_along == PU_BC_SOLID_WALL) P.bc_along = PU_BC_PRECURSOR;   // non-periodic
+  P.wall_z = ctx->cfg.bc_z == PU_BC_SOLID_WALL;
+  P.weno_all = ctx->cfg.weno_all;
+  P.n3 = g.n3(); P.nxh = g.nxh; P.nyh = g.nyh; P.nzh = g.nzh; P.nh = g.nh();
+  double d = dir == 0 ? ctx->cfg.xlen / g.nx : (dir == 1 ? ctx->cfg.ylen / g.ny : ctx->cfg.zlen / g.nz);
+  P.rd = 1. / d;
+  P.hy_rc = hy_cells(ctx, 0); P.hy_tc = hy_cells(ctx, 1);
+  P.hy_re = hy_edges(ctx, 0); P.hy_te = hy_edges(ctx, 1);
+  return P;
+}
+
 // mode: 0 first sweep (writes d_tend), 1 adds to d_tend, 2 adds and combines stage `stage`
 template <int ORD, class T, int DIR>
 static int sweep_one(pu_ctx *ctx, hipStream_t s, int mode, int stage, double dt, int nrelax) {
   const pu::Grid &g = ctx->g;
-  SweepParams P;
-  P.nx = g.nx; P.ny = g.ny; P.nz = g.nz; P.hs = g.hs; P.hp = g.hp; P.ox = g.ox; P.ntr = ctx->cfg.num_tracers;
-  P.bc_along = DIR == 0 ? ctx->cfg.bc_x : (DIR == 1 ? ctx->cfg.bc_y : ctx->cfg.bc_z);
-  if (DIR == 2 && P.bc_along == PU_BC_SOLID_WALL) P.bc_along = PU_BC_PRECURSOR;   // non-periodic
-  P.wall_z = ctx->cfg.bc_z == PU_BC_SOLID_WALL;
-  P.weno_all = ctx->cfg.weno_all;
-  P.n3 = g.n3(); P.nxh = g.nxh; P.nyh = g.nyh; P.nzh = g.nzh; P.nh = g.nh();
-  double d = DIR == 0 ? ctx->cfg.xlen / g.nx : (DIR == 1 ? ctx->cfg.ylen / g.ny : ctx->cfg.zlen / g.nz);
-  P.rd = 1. / d;
-  P.hy_rc = hy_cells(ctx, 0); P.hy_tc = hy_cells(ctx, 1);
-  P.hy_re = hy_edges(ctx, 0); P.hy_te = hy_edges(ctx, 1);
+  SweepParams P = sweep_params(ctx, DIR);
   static const char *names[3] = {"weno_x", "weno_y", "weno_z"};
   LaunchTimer lt(ctx, names[DIR]);
   constexpr int BX = SegShape<DIR>::BX, BY = SegShape<DIR>::BY, NC = SegShape<DIR>::BS - 2;
@@ -376,6 +418,36 @@ int launch_sweeps(pu_ctx *ctx, hipStream_t s, int stage, double dt, int nrelax) 
     case 7: return f32 ? sweeps_t<7, float>(ctx, s, stage, dt, nrelax) : sweeps_t<7, double>(ctx, s, stage, dt, nrelax);
     case 9: return f32 ? sweeps_t<9, float>(ctx, s, stage, dt, nrelax) : sweeps_t<9, double>(ctx, s, stage, dt, nrelax);
     case 11: return f32 ? sweeps_t<11, float>(ctx, s, stage, dt, nrelax) : sweeps_t<11, double>(ctx, s, stage, dt, nrelax);
+  }
+  return fail(PU_ERR_UNSUPPORTED, "unsupported ord");
+}
+
+template <int ORD, class T>
+static int census_t(pu_ctx *ctx, hipStream_t s, unsigned long long *counts) {
+  const pu::Grid &g = ctx->g;
+  dim3 grd((unsigned)((g.nx * g.ny + 255) / 256), (unsigned)g.nz), blk(256);
+  const T *prim = (const T *)ctx->d_prim;
+  hipLaunchKernelGGL((k_flat_census<ORD, T, 0>), grd, blk, 0, s, prim, ctx->d_imm_flag, ctx->d_any10,
+                     sweep_params(ctx, 0), counts);
+  if (g.ny > 1)
+    hipLaunchKernelGGL((k_flat_census<ORD, T, 1>), grd, blk, 0, s, prim, ctx->d_imm_flag, ctx->d_any10,
+                       sweep_params(ctx, 1), counts + 3);
+  hipLaunchKernelGGL((k_flat_census<ORD, T, 2>), grd, blk, 0, s, prim, ctx->d_imm_flag, ctx->d_any10,
+                     sweep_params(ctx, 2), counts + 6);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
+// census of the reconstruction branches on the primitive fields of the last prep; counts: device
+// [3 directions][limiter, flat, linear], zeroed by the caller
+int launch_census(pu_ctx *ctx, hipStream_t s, unsigned long long *counts) {
+  bool f32 = ctx->cfg.precision == PU_FP32;
+  switch (ctx->cfg.ord) {
+    case 3: return f32 ? census_t<3, float>(ctx, s, counts) : census_t<3, double>(ctx, s, counts);
+    case 5: return f32 ? census_t<5, float>(ctx, s, counts) : census_t<5, double>(ctx, s, counts);
+    case 7: return f32 ? census_t<7, float>(ctx, s, counts) : census_t<7, double>(ctx, s, counts);
+    case 9: return f32 ? census_t<9, float>(ctx, s, counts) : census_t<9, double>(ctx, s, counts);
+    case 11: return f32 ? census_t<11, float>(ctx, s, counts) : census_t<11, double>(ctx, s, counts);
   }
   return fail(PU_ERR_UNSUPPORTED, "unsupported ord");
 }

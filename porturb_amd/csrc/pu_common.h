@@ -173,6 +173,7 @@ int launch_sweeps(pu_ctx *ctx, hipStream_t s, int stage = -1, double dt = 0., in
 int launch_convert_c2d(pu_ctx *ctx, const pu_coupler_state *cs, int relax, hipStream_t s);
 int launch_convert_d2c(pu_ctx *ctx, pu_coupler_state *cs, hipStream_t s);
 int launch_tend_export(pu_ctx *ctx, double *tend, hipStream_t s);
+int launch_census(pu_ctx *ctx, hipStream_t s, unsigned long long *counts);
 int dycore_init_device(pu_ctx *ctx, const pu_coupler_state *cs);
 int les_apply_device(pu_ctx *ctx, pu_coupler_state *cs, double dt);
 int les_prim_device(pu_ctx *ctx, const pu_coupler_state *cs);   // closure box d_les only

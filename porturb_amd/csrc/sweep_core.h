@@ -281,6 +281,25 @@ template <int ORD> struct StencilFlags {
       qR = pu_weno::Ops<ORD>::template linR<T>(s);
     }
   }
+
+  // The branch recon<T> takes for the same stencil: 0 the limiter, 1 the flat-stencil branch
+  // (total variation <= 1e-20, evaluated as weno_edges evaluates it), 2 the linear reconstruction
+  // of a cell without WENO.
+  template <class T>
+  __device__ __forceinline__ int branch(const T *p, long long sstride, bool modify) const {
+    if (!use_weno) return 2;
+    using O = pu_weno::Ops<ORD>;
+    p -= (long long)hs * sstride;
+    T s[ORD];
+#pragma unroll
+    for (int q = 0; q < ORD; q++) s[q] = p[(long long)(modify ? clampi(q, lo, hi) : q) * sstride];
+    T TV[O::S];
+    O::template tv<T>(s, TV);
+    T tot = TV[0];
+#pragma unroll
+    for (int m = 1; m < O::S; m++) tot += TV[m];
+    return (PU_FLAT_SHORTCUT && tot <= T(1.e-20)) ? 1 : 0;
+  }
 };
 
 // acoustic upwind state of one face (dynamics_rk.h:640-651): mass flux, pressure, upwind side

@@ -42,13 +42,54 @@ def recon_flops(ord_, do_map=True):
     return f
 
 
-def sweep_flops_per_cell(ord_, nf):
-    """One direction: nf reconstructions, the acoustic Riemann solve at one face, upwinded fluxes
-    and the flux divergence of nf-1 conserved variables."""
+def _tv_flops(ord_):
+    """The candidate stencils' total variations and their sum (evaluated by every branch)."""
+    t = _gen.tables(ord_)
+    N, S = t["N"], t["S"]
+    f = 0
+    for m in range(S):
+        for n in range(1, N):
+            nnz = sum(1 for i in range(N) if t["C"][m][n][i] != 0)
+            f += 2 * nnz - 1
+        for (i, j), c in t["tv"].items():
+            f += 1 if c == 1 else 2
+        f += len(t["tv"]) - 1
+    return f + (S - 1)
+
+
+def linear_flops(ord_):
+    """Both interface values from the full-stencil polynomial (the linear reconstruction)."""
+    t = _gen.tables(ord_)
+    return sum(2 * sum(1 for v in t["lin"][side] if v != 0) - 1 for side in ("L", "R"))
+
+
+def recon_flops_flat(ord_):
+    """Both interface values of a flat stencil (total variation <= 1e-20): the total variations
+    that decide the branch, then the linear reconstruction."""
+    return _tv_flops(ord_) + linear_flops(ord_)
+
+
+def sweep_overhead_flops(nf):
+    """Per cell and direction, besides the reconstructions: the acoustic Riemann solve at one
+    face, the upwinded fluxes and the flux divergence of nf-1 conserved variables."""
     riemann = 16
     advected = (nf - 3) * 1 + 2      # ru_upw * q_upw, hydrostatic theta add-back
     divergence = (nf - 1) * 3
-    return nf * recon_flops(ord_) + riemann + advected + divergence
+    return riemann + advected + divergence
+
+
+def executed_sweep_flops(ord_, nf, census_row, cells):
+    """FLOPs one sweep launch executes over `cells` cells, from the reconstruction census of the
+    state (counts of the limiter, flat-stencil and linear branches)."""
+    n_lim, n_flat, n_lin = (int(v) for v in census_row)
+    return (n_lim * recon_flops(ord_) + n_flat * recon_flops_flat(ord_) + n_lin * linear_flops(ord_)
+            + cells * sweep_overhead_flops(nf))
+
+
+def sweep_flops_per_cell(ord_, nf):
+    """One direction: nf reconstructions, the acoustic Riemann solve at one face, upwinded fluxes
+    and the flux divergence of nf-1 conserved variables."""
+    return nf * recon_flops(ord_) + sweep_overhead_flops(nf)
 
 
 def step_flops_per_cell(ord_, nf):
