@@ -149,6 +149,18 @@ int  pu_geostrophic_wind_forcing(pu_ctx *ctx, pu_coupler_state *cs, double dt, d
 int  pu_level_mean_wind(pu_ctx *ctx, const pu_coupler_state *cs, int k_begin, int k_end, double *u_mean,
                         double *v_mean);
 
+/* Allocation-free variant of pu_level_mean_wind.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight loops, harnesses
+ * that forbid hipMalloc).  It calls no allocating routine: the partial sums live in a static device
+ * array of PU_NOALLOC_MAX_LEVELS levels (256 KiB per GPU, held until the process ends), and it
+ * rejects more levels than that with PU_ERR_ARG.  u_mean, v_mean are DEVICE arrays of
+ * k_end - k_begin doubles; the result is bitwise that of pu_level_mean_wind.  Stream-ordered on
+ * ctx's stream (no synchronisation).  Every call on one GPU shares the array: calls that overlap
+ * on one GPU (other streams, other contexts, other threads) race. */
+#define PU_NOALLOC_MAX_LEVELS 1024
+int  pu_level_mean_wind_noalloc(pu_ctx *ctx, const pu_coupler_state *cs, int k_begin, int k_end, double *u_mean,
+                                double *v_mean);
+
 /* uvel += du, vvel += dv on every cell: the update of uniform_pg_wind_forcing(_height)
  * (uniform_pg_wind_forcing.h:31-34,75-78).  Stream-ordered. */
 int  pu_add_uniform_wind(pu_ctx *ctx, pu_coupler_state *cs, double du, double dv);

@@ -127,6 +127,7 @@ def lib():
         "pu_sponge_layer": (I, [P, csp, D, D, D]),
         "pu_geostrophic_wind_forcing": (I, [P, csp, D, D, D, D]),
         "pu_level_mean_wind": (I, [P, csp, I, I, P, P]),
+        "pu_level_mean_wind_noalloc": (I, [P, csp, I, I, P, P]),
         "pu_add_uniform_wind": (I, [P, csp, D, D]),
         "pu_nc_create": (I, [C.c_char_p, I, C.POINTER(P)]),
         "pu_nc_open": (I, [C.c_char_p, C.POINTER(P)]),
@@ -366,6 +367,11 @@ class Dycore:
         u, v = np.zeros(n), np.zeros(n)
         check(lib().pu_level_mean_wind(self.ctx, C.byref(cs.c), k_begin, k_end, u.ctypes.data, v.ctypes.data))
         return u, v
+
+    def level_mean_wind_noalloc(self, cs, k_begin, k_end, u_dev, v_dev):
+        """pu_level_mean_wind_noalloc: the level means into device pointers u_dev, v_dev
+        (stream-ordered; the library allocates nothing)."""
+        check(lib().pu_level_mean_wind_noalloc(self.ctx, C.byref(cs.c), k_begin, k_end, u_dev, v_dev))
 
     def add_uniform_wind(self, cs, du, dv):
         check(lib().pu_add_uniform_wind(self.ctx, C.byref(cs.c), du, dv))

@@ -91,6 +91,7 @@ int pu_create(const pu_config *cfg, void *stream, pu_ctx **out) {
   pu_ctx *c = new pu_ctx();
   c->cfg = *cfg;
   c->stream = (hipStream_t)stream;
+  if (hipGetDevice(&c->device) != hipSuccess) { delete c; return fail(PU_ERR_HIP, "hipGetDevice failed"); }
   if (getenv("PORTURB_SEGMENT_SWEEPS")) {
     const char *dirs = getenv("PORTURB_SEGMENT_DIRS");      // optional subset, e.g. "z" or "xy"
     for (int d = 0; d < 3; d++) c->segment_sweeps[d] = !dirs || strchr(dirs, "xyz"[d]) != nullptr;

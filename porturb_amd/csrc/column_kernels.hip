@@ -13,6 +13,7 @@
 // streaming pass over the affected levels.
 #include "pu_common.h"
 #include <cmath>
+#include <mutex>
 
 namespace pu {
 
@@ -112,6 +113,39 @@ static int level_means(pu_ctx *ctx, ColFields F, int nfld, int k0, int nlev) {
   return PU_OK;
 }
 
+// Partial sums of the allocation-free level means (pu_level_mean_wind_noalloc): [2][max levels]
+// [COL_CHUNKS] doubles, 256 KiB on every GPU the library runs on, held until the process ends.
+// In host code this name is a host-side stand-in, not a device address: kernels get the address
+// hipGetSymbolAddress returns for the current device (scratch_device).  k_level_partial writes
+// every slot it is asked for, so nothing is cleared between calls; calls that overlap on one GPU
+// share the array and race.
+constexpr int NOALLOC_SLOTS = 2 * PU_NOALLOC_MAX_LEVELS * COL_CHUNKS;
+static __device__ __align__(256) double scratch[NOALLOC_SLOTS];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the context's device, looked up once per device with that device
+// current (the lookup loads the code object; later calls only read the table).
+static int scratch_device(const pu_ctx *ctx, double **out) {
+  const int dev = ctx->device;
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(PU_ERR_ARG, "scratch: device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    PU_HIP_CHECK(hipGetDevice(&cur));
+    if (cur != dev) PU_HIP_CHECK(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) PU_HIP_CHECK(hipSetDevice(cur));
+    if (e != hipSuccess || !p) return fail(PU_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PU_OK;
+}
+
 }  // namespace pu
 
 using namespace pu;
@@ -180,6 +214,38 @@ int pu_level_mean_wind(pu_ctx *ctx, const pu_coupler_state *cs, int k_begin, int
   PU_HIP_CHECK(hipMemcpyAsync(u_mean, ctx->d_col, sizeof(double) * nlev, hipMemcpyDeviceToHost, ctx->stream));
   PU_HIP_CHECK(hipMemcpyAsync(v_mean, ctx->d_col + nlev, sizeof(double) * nlev, hipMemcpyDeviceToHost, ctx->stream));
   PU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return PU_OK;
+}
+
+int pu_level_mean_wind_noalloc(pu_ctx *ctx, const pu_coupler_state *cs, int k_begin, int k_end, double *u_mean,
+                               double *v_mean) {
+  if (!ctx || !cs || !cs->uvel || !cs->vvel || !u_mean || !v_mean)
+    return fail(PU_ERR_ARG, "pu_level_mean_wind_noalloc: null argument");
+  const pu_config &c = ctx->cfg;
+  if (k_begin < 0 || k_end > c.nz || k_begin >= k_end)
+    return fail(PU_ERR_ARG, "pu_level_mean_wind_noalloc: levels [" + std::to_string(k_begin) + ", " +
+                                std::to_string(k_end) + ") outside [0, " + std::to_string(c.nz) + ")");
+  const int nlev = k_end - k_begin;
+  if (nlev > PU_NOALLOC_MAX_LEVELS)
+    return fail(PU_ERR_ARG, "pu_level_mean_wind_noalloc: " + std::to_string(nlev) + " levels exceed the maximum of " +
+                                std::to_string(PU_NOALLOC_MAX_LEVELS) + " (PU_NOALLOC_MAX_LEVELS)");
+  double *part = nullptr;
+  int rc = scratch_device(ctx, &part);
+  if (rc != PU_OK) return rc;
+  // the kernels and sum order of pu_level_mean_wind: partial sums per (field, level, chunk), then
+  // each level's chunks in order; field 0 (u) goes to u_mean, field 1 (v) to v_mean
+  const long long plane = (long long)c.nx * c.ny;
+  ColFields F{{cs->uvel, cs->vvel, nullptr}};
+  hipLaunchKernelGGL(k_level_partial, dim3(COL_CHUNKS, nlev, 2), dim3(256), 0, ctx->stream, F, plane, k_begin, nlev,
+                     part);
+  PU_HIP_CHECK(hipGetLastError());
+  const double scale = 1.0 / ((double)c.nx * c.ny);
+  hipLaunchKernelGGL(k_level_final, dim3((nlev + 255) / 256), dim3(256), 0, ctx->stream, (const double *)part, nlev,
+                     scale, u_mean);
+  PU_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_level_final, dim3((nlev + 255) / 256), dim3(256), 0, ctx->stream,
+                     (const double *)(part + (size_t)nlev * COL_CHUNKS), nlev, scale, v_mean);
+  PU_HIP_CHECK(hipGetLastError());
   return PU_OK;
 }
 

@@ -46,6 +46,7 @@ struct KernelTimer {
 struct pu_ctx {
   pu_config cfg;
   hipStream_t stream = nullptr;
+  int device = 0;                  // the HIP device current when the context was created
   pu::Grid g{};
   int nv = 0;                      // 5 + num_tracers  (conserved variables)
   int nf = 0;                      // nv + 1           (reconstructed fields incl. pressure)
