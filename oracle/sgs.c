@@ -33,6 +33,9 @@
 #define NIN OC_MLP_NIN
 #define NH OC_MLP_NH
 #define NOUT OC_MLP_NOUT
+/* fallback rule: strains / theta gradients below this fraction of the cell's speed / theta per
+ * cell width count as zero (the same constant as the product's inference kernel) */
+#define SGS_NOISE_REL 2e-6
 
 /* offsets of the blocks in the parameter vector */
 #define OFF_W1 0
@@ -255,11 +258,13 @@ int oc_sgs_features(const double *les, int nx, int ny, int nz, const float *fmul
  * A cell falls back to the eddy-viscosity closure (bit cleared) when
  *   - it is in the first or last level, or any of its 27 neighbours has immersed proportion > 0;
  *   - any normalised input exceeds M->xmax in magnitude (out of the training range);
- *   - momentum: the resolved strain is exactly zero (nothing bounds the stresses), or the
- *     predicted dissipation eps = -tau_ij S_ij is negative, or exceeds nu_max * 2 S_ij S_ij (an
+ *   - momentum: the resolved strain is at the rounding level of the cell's speed (S_ij S_ij <=
+ *     (SGS_NOISE_REL (|u| + |v| + |w|) / (2 min(dx, dy, dz)))^2: nothing bounds the stresses), or
+ *     the predicted dissipation eps = -tau_ij S_ij is negative, or exceeds nu_max * 2 S_ij S_ij (an
  *     effective viscosity above the closure's stability cap nu_max);
- *   - heat: the resolved theta gradient is exactly zero, or chi = -q_j dtheta/dx_j is negative or
- *     exceeds nu_max |grad theta|^2.
+ *   - heat: the resolved theta gradient is at the rounding level of theta (|grad theta| <=
+ *     SGS_NOISE_REL theta / (2 min(dx, dy, dz))), or chi = -q_j dtheta/dx_j is negative or exceeds
+ *     nu_max |grad theta|^2.
  * tau: [9][nz][ny][nx] (tau11 tau12 tau13 tau22 tau23 tau33 q1 q2 q3, physical units). */
 void oc_sgs_infer(const oc_sgs_model *M, const double *les, const double *imm, int nx, int ny, int nz, double dx,
                   double dy, double dz, double nu_max, int bf16, float *tau, unsigned char *flags) {
@@ -296,8 +301,11 @@ void oc_sgs_infer(const oc_sgs_model *M, const double *les, const double *imm, i
       double ss = s11 * s11 + s22 * s22 + s33 * s33 + 2 * (s12 * s12 + s13 * s13 + s23 * s23);
       double chi = -(t[6] * tx + t[7] * ty + t[8] * tz);
       double gg = tx * tx + ty * ty + tz * tz;
-      if (ss > 0 && eps >= 0 && eps <= nu_max * 2 * ss) fl |= 1;
-      if (gg > 0 && chi >= 0 && chi <= nu_max * gg) fl |= 2;
+      double rmax = fmax(1 / (2 * dx), fmax(1 / (2 * dy), 1 / (2 * dz)));
+      double smin = SGS_NOISE_REL * (fabs(U[c]) + fabs(V[c]) + fabs(W[c])) * rmax;
+      double gmin = SGS_NOISE_REL * fabs(TH[c]) * rmax;
+      if (ss > smin * smin && eps >= 0 && eps <= nu_max * 2 * ss) fl |= 1;
+      if (gg > gmin * gmin && chi >= 0 && chi <= nu_max * gg) fl |= 2;
     }
     flags[id] = fl;
   }

@@ -34,6 +34,9 @@ constexpr int NIN = PU_SGS_NIN, NINP = 112, NH = PU_SGS_NHIDDEN, NOUT = PU_SGS_N
 constexpr int P_W1 = 0, P_B1 = P_W1 + NH * NINP, P_W2 = P_B1 + NH, P_B2 = P_W2 + NH * NH, P_W3 = P_B2 + NH,
               P_B3 = P_W3 + NOUTP * NH, P_N = P_B3 + NOUTP;
 constexpr int SAMPLES_PER_BLOCK = 64;
+// Fallback rule: resolved strains and theta gradients below this fraction of the cell's speed
+// (theta) per cell width are rounding noise and count as zero (the same constant as oracle/sgs.c).
+constexpr double SGS_NOISE_REL = 2e-6;
 
 typedef short bfx4 __attribute__((ext_vector_type(4)));
 typedef float fx4 __attribute__((ext_vector_type(4)));
@@ -472,20 +475,27 @@ template <class T> struct InferArgs {
 // One group = 64 consecutive cells of an x row (i0..i0+63 at j, k); a block loops over groups.
 // The group's 3x3 neighbourhood rows of (u, v, w, theta), 66 x 3 x 3 points, are staged in LDS
 // with row-contiguous loads; each wave then takes 16 cells: inputs from LDS, the network on the
-// matrix cores, and (lanes 0-15) the outputs and the fallback rule.
+// matrix cores, the outputs stored from the lanes that hold them in the last layer's D fragment,
+// and (lanes 0-15) the fallback rule.
 constexpr int GX = 64, GBX = GX + 2, GBN = GBX * 3 * 3;
 
+// Waves per SIMD the inference kernel is built for (4: 128 VGPRs; 3: up to 168, no spills).
+#ifndef PU_SGS_INFER_WAVES
+#define PU_SGS_INFER_WAVES 3
+#endif
+
 template <class T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sgs_infer(InferArgs<T> A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PU_SGS_INFER_WAVES))) void k_sgs_infer(InferArgs<T> A) {
   constexpr int K1 = 128;                       // layer-1 inputs: 4 K blocks of 32 (108 used)
   __shared__ __align__(16) short sW1[NH * NINP];   // row stride 112: block 3 reads 16 columns
   __shared__ __align__(16) short sW2[NH * NH];  // columns in unit_of_k order
   __shared__ __align__(16) short sW3[NOUTP * NH];
   __shared__ __align__(16) float sB[NH + NH + NOUTP];
+  __shared__ __align__(16) float sMS[2 * NOUTP];   // output mean, output scale (16 slots each)
   __shared__ float sFm[K1];
-  __shared__ float sY[4][16][NOUTP + 1];
   __shared__ T sQ[4 * GBN];                     // u, v, w, theta of the group's neighbourhood
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, r16 = lane & 15, g = lane >> 4;
+  const int tid = threadIdx.x, lane = tid & 63, r16 = lane & 15, g = lane >> 4;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);   // the wave (uniform: row addresses are scalar)
   for (int p = tid; p < NH * NINP / 4; p += 256)
     reinterpret_cast<uint2 *>(sW1)[p] = reinterpret_cast<const uint2 *>(A.wbf + P_W1)[p];
   for (int p = tid; p < NH * NH; p += 256) sW2[p] = A.wbf[P_W2 + (p / NH) * NH + unit_of_k(p % NH)];
@@ -493,10 +503,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   for (int p = tid; p < NH + NH + NOUTP; p += 256)
     sB[p] = p < NH ? A.master[P_B1 + p] : (p < 2 * NH ? A.master[P_B2 + p - NH] : A.master[P_B3 + p - 2 * NH]);
   for (int p = tid; p < K1; p += 256) sFm[p] = p < NIN ? A.fmul[p] : 0.f;
-  const long long ngx = (A.nx + GX - 1) / GX;
-  const long long ngroups = ngx * A.ny * A.nz;
-  for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    const int i0 = (int)(grp % ngx) * GX, j = (int)((grp / ngx) % A.ny), k = (int)(grp / (ngx * A.ny));
+  float *const taug = A.tau + (long long)(4 * g) * A.n3;   // this lane group's first output
+  if (tid < NOUTP) {
+    sMS[tid] = tid < NOUT ? A.out_mean[tid] : 0.f;
+    sMS[NOUTP + tid] = tid < NOUT ? A.out_scale[tid] : 0.f;
+  }
+  // group bookkeeping in 32 bits (the host checks that the group count fits)
+  const unsigned ngx = (unsigned)((A.nx + GX - 1) / GX);
+  const unsigned ngroups = ngx * (unsigned)A.ny * (unsigned)A.nz;
+  for (unsigned grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const unsigned rest = grp / ngx;
+    const int i0 = (int)(grp - rest * ngx) * GX, j = (int)(rest % (unsigned)A.ny), k = (int)(rest / (unsigned)A.ny);
     const int lxmax = (int)(A.nxh - i0);        // box x positions inside the array
     // Staging: the box is 36 rows (quantity q, z offset, y offset) of 66 points; wave w loads rows
     // w, w+4, ... with one lane per x point (lanes 0, 1 also take points 64, 65), so each row's
@@ -511,7 +528,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       buf[rr] = lane < lxmax ? row[lane] : T(0);
       buf2[rr] = (lane < 2 && 64 + lane < lxmax) ? row[64 + lane] : T(0);
     }
-    __syncthreads();                            // the previous group is done with sQ and sY
+    __syncthreads();                            // the previous group is done with sQ
 #pragma unroll
     for (int rr = 0; rr < RPW; rr++) {
       const int r = w + 4 * rr;
@@ -574,15 +591,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     fx4 y = *reinterpret_cast<const fx4 *>(sB + 2 * NH + 4 * g);
 #pragma unroll
     for (int kb = 0; kb < 2; kb++) y = mfma32(ld8(sW3 + r16 * NH + 32 * kb + 8 * g), cat8(a2[2 * kb], a2[2 * kb + 1]), y);
-    for (int r = 0; r < 4; r++) sY[w][r16][4 * g + r] = y[r];
-    __syncthreads();
+    // Lane (r16, g) holds outputs 4 g .. 4 g + 3 of cell r16: denormalise and store them there.
+    const fx4 om = *reinterpret_cast<const fx4 *>(sMS + 4 * g), os = *reinterpret_cast<const fx4 *>(sMS + NOUTP + 4 * g);
+    float t4[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) t4[r] = __fadd_rn(om[r], __fmul_rn(os[r], y[r]));
+    const long long id = ((long long)k * A.ny + j) * A.nx + i;
+    if (in) {
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (4 * g + r < NOUT) taug[r * A.n3 + id] = t4[r];
+    }
+    // outputs 4..7 (lane group 1) and 8 (group 2) to lanes 0..15 for the fallback rule
+    float t[NOUT];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      t[r] = t4[r];
+      t[4 + r] = __shfl(t4[r], lane + 16);
+    }
+    t[8] = __shfl(t4[0], lane + 32);
     if (g == 0 && in) {
-      const long long id = ((long long)k * A.ny + j) * A.nx + i;
-      float t[NOUT];
-      for (int o = 0; o < NOUT; o++) {
-        t[o] = __fadd_rn(A.out_mean[o], __fmul_rn(A.out_scale[o], sY[w][r16][o]));
-        A.tau[o * A.n3 + id] = t[o];
-      }
       unsigned char fl = 0;
       if (A.valid[id] && mx <= A.xmax) {
         const T *U = sQ, *V = sQ + GBN, *W = sQ + 2 * GBN, *TH = sQ + 3 * GBN;
@@ -596,9 +624,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const double ss = s11 * s11 + s22 * s22 + s33 * s33 + 2 * (s12 * s12 + s13 * s13 + s23 * s23);
         const double chi = -(t[6] * tx + t[7] * ty + t[8] * tz);
         const double gg = tx * tx + ty * ty + tz * tz;
-        // a zero strain (gradient) bounds nothing: any stress (flux) gives eps = 0 (chi = 0)
-        if (ss > 0 && eps >= 0 && eps <= A.nu_max * 2 * ss) fl |= 1;
-        if (gg > 0 && chi >= 0 && chi <= A.nu_max * gg) fl |= 2;
+        // A strain (theta gradient) at the rounding level of the cell's own values bounds nothing:
+        // with a zero one any stress (flux) gives eps = 0 (chi = 0), and rounding-level differences
+        // would decide the flag.  Below SGS_NOISE_REL of the cell's speed (theta) per cell width
+        // they count as zero.
+        const double rmax = fmax(A.r2dx, fmax(A.r2dy, A.r2dz));
+        const double smin = SGS_NOISE_REL * (fabs((double)U[c]) + fabs((double)V[c]) + fabs((double)W[c])) * rmax;
+        const double gmin = SGS_NOISE_REL * fabs((double)TH[c]) * rmax;
+        if (ss > smin * smin && eps >= 0 && eps <= A.nu_max * 2 * ss) fl |= 1;
+        if (gg > gmin * gmin && chi >= 0 && chi <= A.nu_max * gg) fl |= 2;
       }
       A.flags[id] = fl;
     }
@@ -698,6 +732,7 @@ static int sgs_infer_t(pu_ctx *ctx, double dt) {
   A.nu_max = 0.5 * std::fmin(vmx, std::fmin(vmy, vmz));
   A.tau = ctx->d_sgs_tau; A.flags = ctx->d_sgs_flags;
   const long long ngroups = (long long)((c.nx + GX - 1) / GX) * c.ny * c.nz;
+  if (ngroups >= (1LL << 32)) return fail(PU_ERR_UNSUPPORTED, "learned closure: grid too large for the inference kernel");
   const unsigned grid = (unsigned)std::min<long long>(ngroups, 256LL * 16);
   hipLaunchKernelGGL(k_sgs_infer<T>, dim3(grid), dim3(256), 0, ctx->stream, A);
   PU_HIP_CHECK(hipGetLastError());

@@ -325,3 +325,28 @@ def test_trained_network_flags_and_outputs_match_oracle():
         err = np.abs(tau[f] - rtau[f])
         assert np.median(err) <= 1e-4 * s and err.max() <= 2e-2 * s, f
     gm.destroy()
+
+
+def test_rounding_level_theta_gradient_falls_back_like_the_oracle():
+    """Levels whose theta is 300 K up to the rounding of the closure's theta = (rho R_d T / C0)^(1/gamma)
+    / rho: the heat flag is cleared on every one of their interior cells, on the GPU and in the
+    oracle alike (the rounding-level gradient is below the fallback rule's noise threshold), while
+    the momentum flags follow the resolved strain there."""
+    cfg = small_config(nx=24, ny=20, nz=14, ord=5, tracers=[WV, TKE, P1, P2], dx=2.)
+    st, imm = random_state(cfg, seed=23, amp_u=1.0, vapor=False)
+    c = pa.physical_constants()
+    rho = st[0]
+    upper = np.arange(cfg["nz"]) >= 7
+    st[4, upper] = c["C0"] * (rho[upper] * 300.) ** c["gamma_d"] / (rho[upper] * c["R_d"])
+    P = np.zeros(o.NPARAM, np.float32)
+    om = np.array([0, 0, -2e-3, 0, 0, 0, 0, 0, -1e-4], np.float32)
+    gm, om_ = model_for(P, om, np.ones(9, np.float32))
+    dt = 0.02
+    _, tau, fl = run_learned(cfg, st, imm, gm, dt)
+    _, rfl = o.les_apply_learned(cfg, st.copy(), imm, dt, om_)
+    # levels 8 .. nz-2 have all 27 neighbours at theta = 300 up to rounding
+    band = fl[8:-1]
+    assert np.all((band & 2) == 0) and np.all((rfl[8:-1] & 2) == 0)
+    assert (band & 1).mean() > 0.2                       # the momentum flags are not affected
+    assert (fl == rfl).mean() >= 0.99
+    gm.destroy()

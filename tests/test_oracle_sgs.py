@@ -261,6 +261,23 @@ def test_fallback_rule_known_answers():
                       lambda k, j, i: 0 * i, lambda k, j, i: 300 + 0.01 * k, lambda k, j, i: 0 * i], nz, ny, nx)
     _, fl6 = o.sgs_infer(const_model([0, 0, 0, 0, 0, 0, 0, 0, -0.001]), still, imm, 1., 1., 1., nu_max)
     assert np.all(fl6[inner & ~near] == 2)
+    # theta constant up to rounding (one-ulp noise on 300 K) and velocities with rounding-level
+    # differences count as zero gradient / strain: fallback, whatever the noise's signs
+    rng = np.random.default_rng(3)
+    ulp = np.spacing(300.)
+    noisy = flat.copy()
+    noisy[4] = 300. + ulp * rng.integers(-2, 3, noisy[4].shape)
+    _, fl7 = o.sgs_infer(const_model([0, 0, -0.01, 0, 0, 0, 0, 0, -0.001]), noisy, imm, 1., 1., 1., nu_max)
+    assert np.all(fl7[inner & ~near] == 1)
+    calm = still.copy()
+    calm[1] = 5. + np.spacing(5.) * rng.integers(-2, 3, calm[1].shape)
+    _, fl8 = o.sgs_infer(const_model([0, 0, -0.01, 0, 0, 0, 0, 0, -0.001]), calm, imm, 1., 1., 1., nu_max)
+    assert np.all(fl8[inner & ~near] == 2)
+    # a gradient just above the threshold (2e-6 theta per cell width) is kept
+    weak = halo_box([lambda k, j, i: 1 + 0 * i, lambda k, j, i: dudz * k, lambda k, j, i: 0 * i,
+                     lambda k, j, i: 0 * i, lambda k, j, i: 300 * (1 + 4e-6 * k), lambda k, j, i: 0 * i], nz, ny, nx)
+    _, fl9 = o.sgs_infer(const_model([0, 0, -0.01, 0, 0, 0, 0, 0, -0.001]), weak, imm, 1., 1., 1., nu_max)
+    assert np.all(fl9[inner & ~near] == 3)
 
 
 def shear_case(n=8, nz=10):
