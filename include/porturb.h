@@ -118,6 +118,13 @@ int  pu_dycore_immersed_masks(pu_ctx *ctx, unsigned char *any10, int *tau);
  * too; this entry point serves contexts that run the closure without the dycore. */
 int  pu_les_closure_init(pu_ctx *ctx, const pu_coupler_state *cs);
 
+/* A caller-held immersed_proportion_halos entry (device [nz+2][ny+2][nx+2]) replaces the one the
+ * context built: the reference's closure and surface fluxes read that DataManager entry at every
+ * apply (les_closure.h:45, surface_flux.h:28), and drivers may edit it (the turbines add their
+ * projection weights, windmill_actuators_yaw.h:159).  Stream-ordered copy; the context keeps its
+ * own copy until the next call. */
+int  pu_les_closure_set_immersed_halos(pu_ctx *ctx, const double *halos);
+
 /* LES_Closure::apply (les_closure.h:30-323).  In place on the coupler state; requires idTKE. */
 int  pu_les_closure_apply(pu_ctx *ctx, pu_coupler_state *cs, double dt_phys);
 

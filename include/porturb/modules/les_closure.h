@@ -45,6 +45,11 @@ struct LES_Closure {
         detail::check(pu_les_closure_set_model(ctx.ctx, ctx.sgs));
       }
     }
+    // a driver-held immersed_proportion_halos entry (les_closure.h:45 reads it at every apply; the
+    // turbines add their projection weights to it) replaces the one the context built
+    auto &dm = coupler.get_data_manager_readonly();
+    if (dm.entry_exists("immersed_proportion_halos"))
+      detail::check(pu_les_closure_set_immersed_halos(ctx.ctx, dm.get<real const, 3>("immersed_proportion_halos").data()));
     detail::check(pu_les_closure_apply(ctx.ctx, &cs, dtphys));
   }
 };

@@ -3,8 +3,9 @@
 // Reads the DataManager entries immersed_roughness, immersed_temp ({nz,ny,nx}) and
 // surface_roughness, surface_temp ({ny,nx}) that custom_modules::sc_init registers
 // (sc_init.h:63-88); the halo'd boxes the reference reads (immersed_*_halos, sc_init.h:728-767)
-// are built inside the shared libporturb context on the first call.  The log-law drag and
-// temperature exchange then run in libporturb's kernel on uvel, vvel, wvel, temp.
+// are built inside the shared libporturb context on the first call; an immersed_proportion_halos
+// entry, when the driver holds one, is read at every call as the reference reads it.  The log-law
+// drag and temperature exchange then run in libporturb's kernel on uvel, vvel, wvel, temp.
 #pragma once
 
 #include "porturb/core/coupler.h"
@@ -36,6 +37,9 @@ inline void apply_surface_fluxes(core::Coupler &coupler, core::real dt) {
                                          dm.get<core::real const, 2>("surface_temp").data()));
     ctx.sfc_ready = true;
   }
+  // a driver-held immersed_proportion_halos entry (surface_flux.h:28 reads it) replaces the context's
+  if (dm.entry_exists("immersed_proportion_halos"))
+    detail::check(pu_les_closure_set_immersed_halos(ctx.ctx, dm.get<core::real const, 3>("immersed_proportion_halos").data()));
   detail::check(pu_surface_fluxes_apply(ctx.ctx, &cs, dt));
 }
 

@@ -89,6 +89,7 @@ def lib():
         "pu_kernel_timing": (I, [P, I]),
         "pu_kernel_time": (I, [P, C.c_char_p, C.POINTER(D), C.POINTER(C.c_longlong)]),
         "pu_les_closure_init": (I, [P, csp]),
+        "pu_les_closure_set_immersed_halos": (I, [P, P]),
         "pu_fill_device": (I, [P, D, S]),
         "pu_memcpy_d2d": (I, [P, P, S]),
         "pu_driver_create": (I, [I, I, I, I, D, I, I, C.c_ulonglong, I, C.c_char_p, C.POINTER(P)]),
@@ -381,6 +382,15 @@ class Dycore:
 
     def les_init(self, cs):
         check(lib().pu_les_closure_init(self.ctx, C.byref(cs.c)))
+
+    def set_immersed_halos(self, halos):
+        """pu_les_closure_set_immersed_halos with a host array [nz+2][ny+2][nx+2] (uploaded)."""
+        halos = np.ascontiguousarray(halos, dtype=np.float64)
+        b = DeviceBuffer(halos.nbytes)
+        b.upload(halos)
+        check(lib().pu_les_closure_set_immersed_halos(self.ctx, b.ptr))
+        self.synchronize()
+        b.free()
 
     def sgs(self, cs, dt):
         """(tau [9][nz][ny][nx] float32, flags [nz][ny][nx] uint8) of the attached network."""

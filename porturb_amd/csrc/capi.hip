@@ -112,7 +112,7 @@ int pu_create(const pu_config *cfg, void *stream, pu_ctx **out) {
   for (int tr = 0; tr < cfg->num_tracers; tr++) nq += cfg->tracer_diffuse[tr] ? 1 : 0;
   if ((rc = alloc(&c->d_hy, sizeof(double) * (3 * (g.nz + 2 * g.hs) + 3 * (g.nz + 1)))) ||
       (rc = alloc(&c->d_imm_flag, nh)) || (rc = alloc(&c->d_any10, n3)) || (rc = alloc(&c->d_tau, n3 * sizeof(int))) ||
-      (rc = alloc(&c->d_relax, n3 * sizeof(double))) || (rc = alloc(&c->d_imm_les_val, nl * sizeof(double))) ||
+      (rc = alloc(&c->d_relax, n3 * sizeof(double))) || (rc = alloc(&c->d_imm_les_val, nl * sizeof(double))) || (rc = alloc(&c->d_imm_les, nl)) ||
       (rc = alloc(&c->d_U, es * c->nv * n3)) || (rc = alloc(&c->d_Us, es * c->nv * n3)) ||
       (rc = alloc(&c->d_tend, es * c->nv * n3)) || (rc = alloc(&c->d_prim, es * c->nf * nh)) ||
       (rc = alloc(&c->d_red, sizeof(double) * (size_t)std::max(4096, 3 * g.nz + 16)))) {
@@ -254,6 +254,15 @@ int pu_les_closure_init(pu_ctx *ctx, const pu_coupler_state *cs) {
   if (rc) return rc;
   ctx->les_ready = true;
   ctx->sgs_valid_ready = false;
+  return PU_OK;
+}
+
+int pu_les_closure_set_immersed_halos(pu_ctx *ctx, const double *halos) {
+  if (!ctx || !halos) return fail(PU_ERR_ARG, "pu_les_closure_set_immersed_halos: null argument");
+  int rc = les_set_immersed_halos_device(ctx, halos);
+  if (rc) return rc;
+  ctx->les_ready = true;
+  ctx->sgs_valid_ready = false;          // the learned closure's mask follows the immersed cells
   return PU_OK;
 }
 

@@ -136,7 +136,10 @@ __global__ __launch_bounds__(256) void k_any_z(const uint8_t *in, uint8_t *out, 
 
 // immersed_proportion_halos for the LES closure (sc_init.h:728-767): halo 1, x/y periodic,
 // bottom z halo 1, top z halo 0.
-__global__ __launch_bounds__(256) void k_imm_les(const double *prop, double *out, int nx, int ny, int nz) {
+// immersed_proportion_halos (values) and its codes for the closure's tile: 0 (no immersed part),
+// 1 (partly immersed), 2 (fully immersed)
+__global__ __launch_bounds__(256) void k_imm_les(const double *prop, double *out, uint8_t *code, int nx, int ny,
+                                                 int nz) {
   long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long nxh = nx + 2, nyh = ny + 2, nzh = nz + 2;
   if (id >= nxh * nyh * nzh) return;
@@ -146,13 +149,33 @@ __global__ __launch_bounds__(256) void k_imm_les(const double *prop, double *out
   else if (kh == nz + 1) v = 0.;
   else v = prop[((long long)(kh - 1) * ny + wrapi_(jh - 1, ny)) * nx + wrapi_(ih - 1, nx)];
   out[id] = v;
+  code[id] = v == 1. ? 2 : (v > 0. ? 1 : 0);
+}
+
+// A caller's immersed_proportion_halos entry in place of the one built from immersed_proportion:
+// values copied, codes rebuilt.
+__global__ __launch_bounds__(256) void k_imm_les_set(const double *src, double *out, uint8_t *code, long long n) {
+  long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const double v = src[id];
+  out[id] = v;
+  code[id] = v == 1. ? 2 : (v > 0. ? 1 : 0);
+}
+
+int les_set_immersed_halos_device(pu_ctx *ctx, const double *halos) {
+  const pu::Grid &g = ctx->g;
+  long long nl = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
+  hipLaunchKernelGGL(k_imm_les_set, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, ctx->stream, halos,
+                     ctx->d_imm_les_val, ctx->d_imm_les, nl);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
 }
 
 int les_init_device(pu_ctx *ctx, const pu_coupler_state *cs) {
   const pu::Grid &g = ctx->g;
   long long nl = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
   hipLaunchKernelGGL(k_imm_les, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, ctx->stream, cs->immersed_proportion,
-                     ctx->d_imm_les_val, g.nx, g.ny, g.nz);
+                     ctx->d_imm_les_val, ctx->d_imm_les, g.nx, g.ny, g.nz);
   PU_HIP_CHECK(hipGetLastError());
   PU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return PU_OK;
@@ -251,7 +274,7 @@ static int build_immersed(pu_ctx *ctx, const double *prop) {
   PU_HIP_CHECK(hipGetLastError());
   long long nl = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
   hipLaunchKernelGGL(k_imm_les, dim3((unsigned)((nl + 255) / 256)), blk, 0, s, prop, ctx->d_imm_les_val,
-                     g.nx, g.ny, g.nz);
+                     ctx->d_imm_les, g.nx, g.ny, g.nz);
   PU_HIP_CHECK(hipGetLastError());
   PU_HIP_CHECK(hipStreamSynchronize(s));
   hipFree(tmp1); hipFree(tmp2);

@@ -24,7 +24,6 @@ struct LesParams {
   double rdelta, nu_pr, rR_d;      // 1 / delta, nu / Pr (Pr = 0.7), 1 / R_d
   double rdx, rdy, rdz;            // 1 / dx, 1 / dy, 1 / dz
   double R_d, C0, gamma;
-  const double *imm;               // immersed_proportion_halos [nzh][nyh][nxh]
   // learned closure: per-cell stresses/heat fluxes [9][n3] and flags [n3] (null: eddy viscosity only)
   const float *tau;
   const unsigned char *sflags;
@@ -97,14 +96,15 @@ static_assert(LTX * LTY * LTZ == 256, "one cell per thread of a 256-thread block
 constexpr int LBX = LTX + 2, LBY = LTY + 2, LBZ = LTZ + 2;   // tile + one-cell halo
 
 template <class T> struct TileBox {
-  const T *sm;
-  int nfl;                                   // fields before the immersed-proportion slot
+  const T *sm;                               // closure fields of the tile + halo, [field][n]
+  const uint8_t *icode;                      // immersed codes of the box points: 0 none, 1 partial, 2 full
   const float *tau;                          // learned closure: global [9][n3] stresses/heat fluxes, or null
   long long n3;
   const unsigned char *flag;                 // learned closure: [n] flags of the box points
   static constexpr int sx = 1, sy = LBX, sz = LBX * LBY, n = LBX * LBY * LBZ;
   __device__ T S(int f, int idx) const { return sm[f * n + idx]; }
-  __device__ T imm(int idx) const { return sm[nfl * n + idx]; }
+  __device__ bool imm_full(int idx) const { return icode[idx] == 2; }   // immersed proportion == 1
+  __device__ bool imm_any(int idx) const { return icode[idx] != 0; }    // immersed proportion > 0
 };
 
 // Face flux on the low face of box cell c (at halo coordinates kh, jh, ih of the closure box) in
@@ -119,7 +119,7 @@ __device__ __forceinline__ void face_flux(const TileBox<T> &B, const LesParams &
   const int sn = DIR == 0 ? sx : (DIR == 1 ? sy : sz);            // normal stride
   const int m = c - sn;                                            // the low neighbour
   auto S = [&](int f, int idx) { return B.S(f, idx); };
-  if (B.imm(m) == T(1) || B.imm(c) == T(1)) { fc.zero = true; return; }
+  if (B.imm_full(m) || B.imm_full(c)) { fc.zero = true; return; }
   fc.zero = false;
   const T h = T(0.5);
   const T rdx = (T)P.rdx, rdy = (T)P.rdy, rdz = (T)P.rdz;
@@ -213,15 +213,17 @@ __device__ __forceinline__ void face_flux(const TileBox<T> &B, const LesParams &
 }
 
 // One block: an LTX x LTY x LTZ tile of cells.  The closure box values of the tile plus a one-cell halo
-// (fields rho_d, u, v, w, theta, K, the diffused tracers and the immersed proportion) are staged in
-// LDS once; each thread then evaluates its cell's six face fluxes, the TKE source and the update.
+// (fields rho_d, u, v, w, theta, K and the diffused tracers, and the box points' immersed codes as
+// bytes) are staged in LDS once; each thread then evaluates its cell's six face fluxes, the TKE
+// source and the update.
 // T = double: the reference's update in FP64 (les_closure.h:280-320, :404-413).  T = float: the
 // tendencies in FP32, applied as increments to the FP64 coupler entries (u += dt tend_u / rho,
 // T *= (1 + dt tend_theta / (rho theta))^gamma, TKE and tracer masses += dt tend), so the entries
 // keep their FP64 resolution.
 template <class T>
-__global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, double *uu, double *vv, double *ww,
-                                                    double *tt, TracerPtrs trc, LesParams P) {
+__global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, const uint8_t *__restrict__ icode,
+                                                    double *uu, double *vv, double *ww, double *tt, TracerPtrs trc,
+                                                    LesParams P) {
   extern __shared__ __align__(16) unsigned char smraw[];
   T *sm = reinterpret_cast<T *>(smraw);
   const int nfl = 6 + P.nq;
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, dou
     goff[e] = ((long long)kh * P.nyh + jh) * P.nxh + ih;
   }
   constexpr int FB = 4;
-  for (int f0 = 0; f0 < nfl + 1; f0 += FB) {
+  for (int f0 = 0; f0 < nfl; f0 += FB) {
     T buf[FB][PER];
 #pragma unroll
     for (int df = 0; df < FB; df++)
@@ -251,20 +253,31 @@ __global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, dou
       for (int e = 0; e < PER; e++) {
         const int f = f0 + df;
         buf[df][e] = T(0);
-        if (f <= nfl && gin[e]) buf[df][e] = f < nfl ? F[(long long)f * P.nh + goff[e]] : (T)P.imm[goff[e]];
+        if (f < nfl && gin[e]) buf[df][e] = F[(long long)f * P.nh + goff[e]];
       }
 #pragma unroll
     for (int df = 0; df < FB; df++)
 #pragma unroll
       for (int e = 0; e < PER; e++) {
         const int q = threadIdx.x + 256 * e;
-        if (f0 + df <= nfl && q < NB) sm[(f0 + df) * NB + q] = buf[df][e];
+        if (f0 + df < nfl && q < NB) sm[(f0 + df) * NB + q] = buf[df][e];
       }
+  }
+  uint8_t *sic = reinterpret_cast<uint8_t *>(sm + NB * nfl);
+  {
+    uint8_t ib[PER];
+#pragma unroll
+    for (int e = 0; e < PER; e++) ib[e] = gin[e] ? icode[goff[e]] : (uint8_t)0;
+#pragma unroll
+    for (int e = 0; e < PER; e++) {
+      const int q = threadIdx.x + 256 * e;
+      if (q < NB) sic[q] = ib[e];
+    }
   }
   unsigned char *sflag = nullptr;
   if (P.sflags) {
     // the network's flags of the box points (interior cells; periodic boundaries wrap)
-    sflag = reinterpret_cast<unsigned char *>(sm + NB * (nfl + 1));
+    sflag = sic + NB;
     unsigned char fb[PER];
 #pragma unroll
     for (int e = 0; e < PER; e++) {
@@ -289,7 +302,7 @@ __global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, dou
   if (i >= P.nx || j >= P.ny || k >= P.nz) return;
   const long long id = ((long long)k * P.ny + j) * P.nx + i;
   const int ih = i + 1, jh = j + 1, kh = k + 1;
-  const TileBox<T> B{sm, nfl, P.sflags ? P.tau : nullptr, P.n3, sflag};
+  const TileBox<T> B{sm, sic, P.sflags ? P.tau : nullptr, P.n3, sflag};
   constexpr int sx = TileBox<T>::sx, sy = TileBox<T>::sy, sz = TileBox<T>::sz;
   const int c = ((lk + 1) * LBY + (lj + 1)) * LBX + (li + 1);
   auto S = [&](int f, int idx) { return B.S(f, idx); };
@@ -330,14 +343,13 @@ __global__ __launch_bounds__(256) void k_les_update(const T *__restrict__ F, dou
   const T ell = fmin(T(0.76) * sK * recip(N + T(1.e-20)), (T)P.delta);
   const T km = T(0.1) * ell * sK;
   T src = T(0);
-  auto imm = [&](int idx) { return B.imm(idx); };
-  if (imm(c) < T(1)) {
+  if (!B.imm_full(c)) {
     // -(g rho km) / (t Pr_t) dt_dz with 1 / Pr_t = (1 + 2 ell) / delta
     if (P.enable_gravity) src += -((T)P.grav * rho * km) * (rt_ * (T(1) + T(2) * ell) * rdelta) * dt_dz;
     src -= (T)P.diss_mult * rho * (T(0.19) + T(0.51) * ell * rdelta) * rdelta * (K * sK);
-    int im1 = imm(c - sx) > T(0) ? c : c - sx, ip1 = imm(c + sx) > T(0) ? c : c + sx;
-    int jm1 = imm(c - sy) > T(0) ? c : c - sy, jp1 = imm(c + sy) > T(0) ? c : c + sy;
-    int km1 = imm(c - sz) > T(0) ? c : c - sz, kp1 = imm(c + sz) > T(0) ? c : c + sz;
+    int im1 = B.imm_any(c - sx) ? c : c - sx, ip1 = B.imm_any(c + sx) ? c : c + sx;
+    int jm1 = B.imm_any(c - sy) ? c : c - sy, jp1 = B.imm_any(c + sy) ? c : c + sy;
+    int km1 = B.imm_any(c - sz) ? c : c - sz, kp1 = B.imm_any(c + sz) ? c : c + sz;
     const T du_dx = (S(1, c + sx) - S(1, c - sx)) * r2dx;
     const T dv_dx = (S(2, ip1) - S(2, im1)) * r2dx;
     const T dw_dx = (S(3, ip1) - S(3, im1)) * r2dx;
@@ -418,7 +430,6 @@ static int les_params(pu_ctx *ctx, const pu_coupler_state *cs, double dt, LesPar
   P.dt = dt;
   P.visc_max_x = 0.25 * P.dx * P.dx / dt; P.visc_max_y = 0.25 * P.dy * P.dy / dt; P.visc_max_z = 0.25 * P.dz * P.dz / dt;
   P.R_d = c.R_d; P.C0 = c.C0; P.gamma = c.gamma_d;
-  P.imm = ctx->d_imm_les_val;
   P.tau = nullptr;
   P.sflags = nullptr;
   for (int tr = 0; tr < PU_MAX_TRACERS; tr++) {
@@ -440,12 +451,12 @@ static int launch_les_prim(pu_ctx *ctx, const pu_coupler_state *cs, const LesPar
 template <class T>
 static int launch_les_update(pu_ctx *ctx, pu_coupler_state *cs, const LesParams &P, const TracerPtrs &tp) {
   const dim3 grd((unsigned)((P.nx + LTX - 1) / LTX), (unsigned)((P.ny + LTY - 1) / LTY), (unsigned)((P.nz + LTZ - 1) / LTZ));
-  // box fields (<= 119 KB for 16 tracers in FP64) + the network's outputs and flags (24 KB)
-  const size_t lds = sizeof(T) * TileBox<T>::n * (7 + P.nq) + (P.sflags ? TileBox<T>::n : 0);
+  // box fields (<= 114 KB for 16 tracers in FP64), the immersed codes and the network's flags
+  const size_t lds = sizeof(T) * TileBox<T>::n * (6 + P.nq) + TileBox<T>::n * (P.sflags ? 2 : 1);
   if (lds > 65536)
     PU_HIP_CHECK(hipFuncSetAttribute((const void *)k_les_update<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_les_update<T>, grd, dim3(256), lds, ctx->stream, (const T *)ctx->d_les, cs->uvel, cs->vvel,
-                     cs->wvel, cs->temp, tp, P);
+  hipLaunchKernelGGL(k_les_update<T>, grd, dim3(256), lds, ctx->stream, (const T *)ctx->d_les,
+                     (const uint8_t *)ctx->d_imm_les, cs->uvel, cs->vvel, cs->wvel, cs->temp, tp, P);
   PU_HIP_CHECK(hipGetLastError());
   return PU_OK;
 }

@@ -152,6 +152,33 @@ def test_les_matches_oracle():
     assert nrm.max() < 1e-12, nrm
 
 
+def test_les_reads_a_caller_immersed_halos_entry():
+    """A driver-held immersed_proportion_halos entry (here: the context's own plus one more fully
+    immersed cell and a partly immersed one) replaces the context's: the closure then matches the
+    oracle run on the immersed proportion those halos describe, and differs from the run without."""
+    cfg = small_config(nx=22, ny=20, nz=16, ord=5, tracers=[WV, TKE, P1, P2])
+    st, imm = random_state(cfg, seed=9, buildings=2)
+    imm2 = imm.copy()
+    imm2[5, 7, 11] = 1.
+    imm2[9, 3, 2] = 0.4
+    nz, ny, nx = imm.shape
+    halos = np.zeros((nz + 2, ny + 2, nx + 2))
+    halos[0] = 1.                                           # bottom halo 1, top halo 0
+    jj = (np.arange(ny + 2) - 1) % ny
+    ii = (np.arange(nx + 2) - 1) % nx
+    halos[1:nz + 1] = imm2[:, jj][:, :, ii]                 # x/y periodic
+    d, cs = gpu_setup(cfg, st, imm)
+    d.set_immersed_halos(halos)
+    d.les_apply(cs, 0.3)
+    ref = st.copy()
+    oracle_py.les_apply(cfg, ref, imm2, 0.3)
+    nrm = rel_norms(cs.download(), ref)
+    assert nrm.max() < 1e-12, nrm
+    plain = st.copy()
+    oracle_py.les_apply(cfg, plain, imm, 0.3)
+    assert np.abs(plain[1] - ref[1]).max() > 1e-6
+
+
 def test_les_fp32_matches_oracle():
     """FP32 dycore mode: the closure's box, fluxes and tendencies are float, applied as increments
     to the FP64 entries.  Each field's increment agrees with the FP64 oracle's to 1e-4 of its
