@@ -8,11 +8,19 @@
  * TransformMatrices.h:105,429).
  *
  * Deliberate, documented differences from the as-written reference:
- *  - hydrostatic column means are accumulated in double (reference: float, dynamics_rk.h:961-975);
+ *  - hydrostatic column means are accumulated in double with compensated (TwoSum) additions
+ *    (reference: float running sums, dynamics_rk.h:961-975);
  *  - the immersed tau box scan (dynamics_rk.h:389-425) clamps indices into the halo'd array
  *    (the reference reads out of bounds for ord 5, whose halo is 2 but the scan radius is 3).
  */
 #include "oracle.h"
+
+/* s + x with the rounding error of the addition added to *c (Knuth's TwoSum) */
+static void two_sum_add(double *s, double *c, double x) {
+  double t = *s + x, bb = t - *s;
+  *c += (*s - (t - bb)) + (x - bb);
+  *s = t;
+}
 
 #include <math.h>
 #include <stdlib.h>
@@ -375,13 +383,18 @@ oc_dycore *oc_dycore_create(const oc_config *cfg, const double *coupler, const d
   double *p = d->hy_pressure_cells = (double *)calloc(nzh, sizeof(double));
 #pragma omp parallel for
   for (int k = hs; k < hs + nz; k++) {
-    double sr = 0, st = 0, sp = 0;
+    /* compensated sums (TwoSum of each addition, the rounding errors summed apart): accurate to
+     * about one rounding in any order, where a running sum of a level's equal densities drifts by
+     * ~n eps */
+    double sr = 0, st = 0, sp = 0, cr = 0, ct = 0, cp = 0;
     for (int j = 0; j < ny; j++) for (int i = 0; i < nx; i++) {
       double rho = H3(state + idR * nh, k, hs + j, hs + i), rt = H3(state + idT * nh, k, hs + j, hs + i);
-      sr += rho; st += rt / rho; sp += C0 * pow(rt, gamma);
+      two_sum_add(&sr, &cr, rho);
+      two_sum_add(&st, &ct, rt / rho);
+      two_sum_add(&sp, &cp, C0 * pow(rt, gamma));
     }
     double rn = 1. / ((double)nx * (double)ny);
-    r[k] = sr * rn; tt[k] = st * rn; p[k] = sp * rn;
+    r[k] = (sr + cr) * rn; tt[k] = (st + ct) * rn; p[k] = (sp + cp) * rn;
   }
   /* isentropic halo extrapolation (dynamics_rk.h:985-1008) */
   for (int kk = 0; kk < hs; kk++) {

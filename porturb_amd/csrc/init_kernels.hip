@@ -18,12 +18,28 @@ struct HydroParams {
   unsigned int adds_mass_mask;
 };
 
+// Compensated sum (Knuth's TwoSum of every addition, the rounding errors summed apart): the level
+// sums below are accurate to about one rounding whatever the order of their terms, so the
+// hydrostatic profiles do not depend on how the cells are split over threads (a plain running sum
+// of a level's 2^20 equal densities drifts by ~1e-13, which the split into hydrostatic and
+// perturbation pressure turns into ~1e-8 Pa).
+struct CSum {
+  double s = 0, c = 0;
+  __device__ void add(double x) {
+    const double t = s + x, bb = t - s;
+    c += (s - (t - bb)) + (x - bb);
+    s = t;
+  }
+  __device__ void add(const CSum &o) { add(o.s); c += o.c; }
+  __device__ double value() const { return s + c; }
+};
+
 __global__ __launch_bounds__(256) void k_hydro_means(const double *rd_, const double *tt, TracerPtrsC trc,
                                                      double *out, HydroParams P) {
-  __shared__ double sr[256], st[256], sp[256];
+  __shared__ double sr[2][256], st[2][256], sp[2][256];
   int k = blockIdx.x;
   long long plane = (long long)P.nx * P.ny;
-  double ar = 0, at = 0, ap = 0;
+  CSum ar, at, ap;
   for (long long q = threadIdx.x; q < plane; q += blockDim.x) {
     long long id = (long long)k * plane + q;
     double rho_d = rd_[id], temp = tt[id];
@@ -32,19 +48,26 @@ __global__ __launch_bounds__(256) void k_hydro_means(const double *rd_, const do
     double rho = rho_d;
     for (int tr = 0; tr < P.ntr; tr++) if ((P.adds_mass_mask >> tr) & 1u) rho += trc.p[tr][id];
     double rt = pow(press / P.C0, 1. / P.gamma);         // rho*theta
-    ar += rho; at += rt / rho; ap += P.C0 * pow(rt, P.gamma);
+    ar.add(rho); at.add(rt / rho); ap.add(P.C0 * pow(rt, P.gamma));
   }
-  sr[threadIdx.x] = ar; st[threadIdx.x] = at; sp[threadIdx.x] = ap;
+  const int t = threadIdx.x;
+  sr[0][t] = ar.s; sr[1][t] = ar.c; st[0][t] = at.s; st[1][t] = at.c; sp[0][t] = ap.s; sp[1][t] = ap.c;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      sr[threadIdx.x] += sr[threadIdx.x + w];
-      st[threadIdx.x] += st[threadIdx.x + w];
-      sp[threadIdx.x] += sp[threadIdx.x + w];
+    if (t < w) {
+      CSum a{sr[0][t], sr[1][t]}, b{st[0][t], st[1][t]}, c{sp[0][t], sp[1][t]};
+      a.add(CSum{sr[0][t + w], sr[1][t + w]});
+      b.add(CSum{st[0][t + w], st[1][t + w]});
+      c.add(CSum{sp[0][t + w], sp[1][t + w]});
+      sr[0][t] = a.s; sr[1][t] = a.c; st[0][t] = b.s; st[1][t] = b.c; sp[0][t] = c.s; sp[1][t] = c.c;
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) { out[k] = sr[0]; out[P.nz + k] = st[0]; out[2 * P.nz + k] = sp[0]; }
+  if (t == 0) {
+    out[k] = CSum{sr[0][0], sr[1][0]}.value();
+    out[P.nz + k] = CSum{st[0][0], st[1][0]}.value();
+    out[2 * P.nz + k] = CSum{sp[0][0], sp[1][0]}.value();
+  }
 }
 
 // ------------------------------------------------------------------------------------------------

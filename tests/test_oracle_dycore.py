@@ -168,3 +168,21 @@ def test_immersed_masks():
                 ref[k, j, i] = (imm[ks][:, jj][:, :, ii] > 0).any()
     assert (any10 == ref).all()
     assert tau[0, 0, 0] == 8 and tau[3, 9, 9] == 1 and tau[5, 9, 9] == 2 and tau[6, 9, 9] == 4
+
+
+def test_hydrostatic_level_means_are_compensated_sums():
+    """A level of 2^15 equal densities and equal theta has exactly that density and theta as its
+    mean (to an ulp): the column means are compensated sums.  A running sum of the same terms
+    drifts by hundreds of ulps (~1e-13), which the hydrostatic pressure turns into ~1e-8 Pa."""
+    from tests import pin_cases as pc
+    cfg = pc.cube_config((256, 128, 6), 5., 5, [WV])
+    _, _, z = pc.coords(cfg)
+    rho = isentropic(z)[:, None, None] * np.ones((6, 128, 256))
+    st = pc.coupler_state(cfg, rho, 7., 0., 0., 300.)
+    d = oracle_py.Dycore(cfg, st, np.zeros(rho.shape))
+    rc = d.hydrostatics()[0]
+    hs = 2
+    ulp = np.spacing(rho[:, 0, 0])
+    assert np.all(np.abs(rc[hs:hs + 6] - rho[:, 0, 0]) <= 2 * ulp)
+    naive = np.array([np.cumsum(np.full(256 * 128, r))[-1] / (256 * 128) for r in rho[:, 0, 0]])
+    assert np.abs(naive - rho[:, 0, 0]).max() > 50 * ulp.max()      # what a running sum would give
