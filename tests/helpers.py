@@ -82,6 +82,18 @@ def rel_norms(a, b, scale=None):
     return np.array(out)
 
 
+def rounding_sensitivity(run, st, ref, rel=1e-15, seed=0):
+    """Per-field relative [L1, L2, Linf] by which `run` (a function of an initial state, e.g. N oracle
+    steps) moves when the initial temperature is perturbed by `rel` relative (random signs), against
+    `ref` = run(st): how far a rounding-level difference carries in this case.  Resting air is
+    ill-conditioned under the WENO limiter: a stencil whose candidates' total variation crosses the
+    1e-20 guard switches from the ideal to the nonlinear weights (WenoLimiter.h convexify), so
+    velocities of ~1e-10 m/s at the edge of the resting region change by O(1) of themselves."""
+    s = st.copy()
+    s[4] = s[4] * (1 + rel * np.random.default_rng(seed).standard_normal(s[4].shape))
+    return rel_norms(run(s), ref)
+
+
 def small_config(nx=20, ny=18, nz=16, ord=5, tracers=None, precision=PU_FP64, dx=50., **kw):
     if tracers is None:
         tracers = [WV, TKE, P1]

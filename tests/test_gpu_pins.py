@@ -2,8 +2,10 @@
 each case is checked against its closed form directly and against the oracle on the same input.
 
 Bounds: FP64 tendencies 1e-11 of the largest tendency vs the oracle, time steps 1e-10 relative per
-field; the closed forms as in the oracle's pins.  FP32 runs are held to the closed forms with the
-bounds stated at each test (float rounding of the stencil and flux arithmetic).
+field (the warm bubble's velocities: within the oracle's own response to a rounding-level
+perturbation, see tests.helpers.rounding_sensitivity); the closed forms as in the oracle's pins.
+FP32 runs are held to the closed forms with the bounds stated at each test (float rounding of the
+stencil and flux arithmetic).
 """
 import numpy as np
 import pytest
@@ -11,7 +13,7 @@ import pytest
 import porturb_amd as pa
 from oracle import oracle_py
 from tests import pin_cases as pc
-from tests.helpers import rel_norms
+from tests.helpers import rel_norms, rounding_sensitivity
 
 pytestmark = pytest.mark.gpu
 
@@ -105,7 +107,10 @@ def test_denser_cell_sinks(ord_, precision):
     pc.check_buoyancy_tendencies(cfg, st, g, cell, drho, tol=1e-9 if precision == pa.PU_FP64 else 1e-3)
     if precision == pa.PU_FP64:
         o = oracle_py.Dycore(cfg, st, imm).tendencies(st)
-        assert np.abs(g - o).max() < 1e-11 * 9.81 * drho
+        # at rest the W tendency is the difference of hydrostatic edge pressures (~1e5 Pa, FP64
+        # rounding 1e-11 Pa each) over dz, balanced against gravity: rounding of 1e-14 p / dz
+        dz = cfg["zlen"] / cfg["nz"]
+        assert np.abs(g - o).max() < 1e-14 * 1e5 / dz
 
 
 def test_warm_bubble_rises():
@@ -113,7 +118,15 @@ def test_warm_bubble_rises():
     print("warm bubble 24^3", flush=True)
     g, dt = gpu_steps(cfg, st, imm, 10)
     pc.check_bubble_rises(cfg, g)
-    assert rel_norms(g, oracle_steps(cfg, st, imm, 10, dt)).max() < 1e-10
+    ref = oracle_steps(cfg, st, imm, 10, dt)
+    nrm = rel_norms(g, ref)
+    # The resting air around the bubble is ill-conditioned under the limiter's 1e-20 guard: the
+    # oracle itself moves by ~5e-8 in u, v when its input is perturbed by 1e-15.  The GPU must stay
+    # within that rounding response; the well-conditioned rho and theta within 1e-10.
+    sens = rounding_sensitivity(lambda s: oracle_steps(cfg, s, imm, 10, dt), st, ref)
+    print("GPU vs oracle:\n", nrm, "\noracle vs 1e-15-perturbed oracle:\n", sens, flush=True)
+    assert (nrm <= 10 * sens + 1e-12).all()
+    assert nrm[[0, 4]].max() < 1e-10
 
 
 @pytest.mark.parametrize("precision", [pa.PU_FP64, pa.PU_FP32], ids=["fp64", "fp32"])
