@@ -348,7 +348,7 @@ int  pu_driver_destroy(pu_driver *d);
  * Learned subgrid closure.  No counterpart in the reference (its nearest prior art is the
  * explicit-vs-closure SGS comparison in build/postproc/compare_les.py:11-60 and the empty
  * external/ponni inference submodule, model/CMakeLists.txt:22).  A per-cell network
- * 108 -> 64 -> 64 -> 9 (GELU) maps the differences of (u, v, w, theta) over a cell's 3x3x3
+ * 108 -> 64 -> 64 -> 9 (ReLU) maps the differences of (u, v, w, theta) over a cell's 3x3x3
  * neighbourhood to the subgrid stresses tau_11 tau_12 tau_13 tau_22 tau_23 tau_33 and heat
  * fluxes q_1 q_2 q_3; attached to a context it replaces LES_Closure's eddy-viscosity momentum and
  * theta face fluxes (les_closure.h:92-229) on every face whose two cells hold the network's flag,

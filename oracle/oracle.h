@@ -119,8 +119,8 @@ typedef struct {
 } oc_sgs_model;
 
 float  oc_bf16_round(float x);
-double oc_gelu(double x);
-double oc_gelu_deriv(double x);
+double oc_relu(double x);
+double oc_relu_deriv(double x);
 void   oc_mlp_forward(const float *P, const float *X, long n, int bf16, float *Y);
 double oc_mlp_loss_grad(const float *P, const float *X, const float *T, long n, int bf16, double *G);
 void   oc_mlp_adam(float *P, const float *G, float *m, float *v, long np, int t, float lr, float b1, float b2,

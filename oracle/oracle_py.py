@@ -85,10 +85,10 @@ def lib():
         F, LG = C.c_float, C.c_long
         L.oc_bf16_round.argtypes = [F]
         L.oc_bf16_round.restype = F
-        L.oc_gelu.argtypes = [D]
-        L.oc_gelu.restype = D
-        L.oc_gelu_deriv.argtypes = [D]
-        L.oc_gelu_deriv.restype = D
+        L.oc_relu.argtypes = [D]
+        L.oc_relu.restype = D
+        L.oc_relu_deriv.argtypes = [D]
+        L.oc_relu_deriv.restype = D
         L.oc_mlp_forward.argtypes = [P, P, LG, I, P]
         L.oc_mlp_forward.restype = None
         L.oc_mlp_loss_grad.argtypes = [P, P, P, LG, I, P]

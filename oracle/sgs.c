@@ -13,7 +13,7 @@
  *   oc_sgs_features    the 108 inputs of one cell: for the 27 cells of its 3x3x3 neighbourhood
  *                      (n = (dz+1)*9 + (dy+1)*3 + (dx+1)) the differences of (u, v, w, theta)
  *                      to the centre cell, times a per-input multiplier fmul (= 1/(delta*sigma));
- *   oc_mlp_forward     108 -> 64 -> 64 -> 9, GELU (erf form) on the hidden layers;
+ *   oc_mlp_forward     108 -> 64 -> 64 -> 9, ReLU on the hidden layers;
  *   oc_mlp_loss_grad   mean-squared loss over n samples and its exact gradient (backpropagation);
  *   oc_mlp_adam        one Adam step (Kingma & Ba 2015, Algorithm 1, bias-corrected);
  *   oc_sgs_infer       prediction + fallback decision for every interior cell of a halo-1
@@ -57,11 +57,10 @@ float oc_bf16_round(float x) {
 
 static inline float rnd(float x, int bf16) { return bf16 ? oc_bf16_round(x) : x; }
 
-double oc_gelu(double x) { return 0.5 * x * (1. + erf(x * 0.70710678118654752440)); }
+/* ReLU and its derivative (0 at x = 0, as torch's backward takes it) */
+double oc_relu(double x) { return x > 0. ? x : 0.; }
 
-double oc_gelu_deriv(double x) {
-  return 0.5 * (1. + erf(x * 0.70710678118654752440)) + x * exp(-0.5 * x * x) * 0.39894228040143267794;
-}
+double oc_relu_deriv(double x) { return x > 0. ? 1. : 0.; }
 
 /* One sample through the network.  Keeps the pre-activations (float, as the GPU holds them) and
  * the rounded activations for the backward pass. */
@@ -78,13 +77,13 @@ static void forward_one(const float *P, const float *xin, int bf16, fwd_t *f) {
     double s = 0;
     for (int i = 0; i < NIN; i++) s += (double)rnd(P[OFF_W1 + o * NIN + i], bf16) * f->x[i];
     f->h1[o] = (float)s + P[OFF_B1 + o];
-    f->a1[o] = rnd((float)oc_gelu(f->h1[o]), bf16);
+    f->a1[o] = rnd((float)oc_relu(f->h1[o]), bf16);
   }
   for (int o = 0; o < NH; o++) {
     double s = 0;
     for (int i = 0; i < NH; i++) s += (double)rnd(P[OFF_W2 + o * NH + i], bf16) * f->a1[i];
     f->h2[o] = (float)s + P[OFF_B2 + o];
-    f->a2[o] = rnd((float)oc_gelu(f->h2[o]), bf16);
+    f->a2[o] = rnd((float)oc_relu(f->h2[o]), bf16);
   }
   for (int o = 0; o < NOUT; o++) {
     double s = 0;
@@ -126,7 +125,7 @@ double oc_mlp_loss_grad(const float *P, const float *X, const float *T, long n, 
     for (int i = 0; i < NH; i++) {
       double s3 = 0;
       for (int o = 0; o < NOUT; o++) s3 += (double)rnd(P[OFF_W3 + o * NH + i], bf16) * dyr[o];
-      dh2[i] = (float)((double)(float)s3 * oc_gelu_deriv(f.h2[i]));
+      dh2[i] = (float)((double)(float)s3 * oc_relu_deriv(f.h2[i]));
       dh2r[i] = rnd(dh2[i], bf16);
     }
     /* layer 2 */
@@ -138,7 +137,7 @@ double oc_mlp_loss_grad(const float *P, const float *X, const float *T, long n, 
     for (int i = 0; i < NH; i++) {
       double s2 = 0;
       for (int o = 0; o < NH; o++) s2 += (double)rnd(P[OFF_W2 + o * NH + i], bf16) * dh2r[o];
-      dh1[i] = (float)((double)(float)s2 * oc_gelu_deriv(f.h1[i]));
+      dh1[i] = (float)((double)(float)s2 * oc_relu_deriv(f.h1[i]));
       dh1r[i] = rnd(dh1[i], bf16);
     }
     /* layer 1 */

@@ -1,6 +1,6 @@
 // sgs_kernels.hip -- learned subgrid closure on gfx950 (no counterpart in the reference).
 //
-// A per-cell network 108 -> 64 -> 64 -> 9 (GELU) predicts the subgrid stresses and heat fluxes of
+// A per-cell network 108 -> 64 -> 64 -> 9 (ReLU) predicts the subgrid stresses and heat fluxes of
 // a cell from the differences of (u, v, w, theta) over its 3x3x3 neighbourhood.  The network runs
 // on the matrix cores: v_mfma_f32_16x16x16_bf16, BF16 operands, FP32 accumulation.
 //
@@ -69,31 +69,9 @@ __host__ __device__ constexpr int unit_of_k(int k) {
   return 16 * (2 * (k / 32) + ((k % 8) >= 4 ? 1 : 0)) + 4 * ((k % 32) / 8) + (k % 4);
 }
 
-// GELU(x) = x Phi(x), Phi(x) = (1 + erf(x / sqrt 2)) / 2, with erf from Abramowitz & Stegun,
-// Handbook of Mathematical Functions, formula 7.1.26 (|error| <= 1.5e-7, far below the BF16
-// rounding of the activations): for z >= 0, erf(z) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5))))
-// exp(-z^2), t = 1 / (1 + p z).  phi_tail returns E = (1 - erf(|x| / sqrt 2)) / 2 (the factors
-// 1/sqrt 2 and 1/2 folded into p and the a_i), so Phi = 1 - E for x >= 0 and E for x < 0, and
-// GELU(x) = max(x, 0) - |x E| without a branch or select.
-__device__ __forceinline__ float phi_tail(float x, float &ex) {
-  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, fabsf(x), 1.f));
-  const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 0.5f * 1.061405429f, 0.5f * -1.453152027f), 0.5f * 1.421413741f),
-                                      0.5f * -0.284496736f),
-                             0.5f * 0.254829592f);
-  ex = __builtin_amdgcn_exp2f(x * x * (-0.5f * 1.44269504088896341f));  // exp(-x^2 / 2)
-  return poly * ex;
-}
-__device__ __forceinline__ float gelu(float x) {
-  float ex;
-  const float e = phi_tail(x, ex);
-  return fmaxf(x, 0.f) - fabsf(x * e);
-}
-__device__ __forceinline__ float gelu_deriv(float x) {                   // Phi(x) + x phi(x)
-  float ex;
-  const float e = phi_tail(x, ex);
-  const float phi = x >= 0.f ? 1.f - e : e;
-  return fmaf(x * 0.39894228040143268f, ex, phi);
-}
+// Hidden-layer activation: ReLU, and its derivative (0 at x = 0).
+__device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.f); }
+__device__ __forceinline__ float relu_deriv(float x) { return x > 0.f ? 1.f : 0.f; }
 
 // sum over the 16 lanes that share l/16 (one column group of a D fragment)
 __device__ __forceinline__ float sum16(float v) {
@@ -183,7 +161,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs A) {
       const int u = 16 * mt + 4 * g + r;
       const float h = acc[r] + A.master[P_B1 + u];
       h1[mt][r] = h;
-      sA1[my * NH + u] = bf16_bits(gelu(h));
+      sA1[my * NH + u] = bf16_bits(relu(h));
     }
   }
   __syncthreads();
@@ -195,7 +173,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs A) {
       const int u = 16 * mt + 4 * g + r;
       const float h = acc[r] + A.master[P_B2 + u];
       h2[mt][r] = h;
-      sA2[my * NH + u] = bf16_bits(gelu(h));
+      sA2[my * NH + u] = bf16_bits(relu(h));
     }
   }
   __syncthreads();
@@ -235,7 +213,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs A) {
     if (lane == 0) sLoss[w] = v;
   }
   __syncthreads();
-  // dA2 = W3^T dy, dH2 = dA2 * gelu'(h2)
+  // dA2 = W3^T dy, dH2 = dA2 * relu'(h2)
   for (int mt = 0; mt < 4; mt++) {
     bfx4 a;
     for (int q = 0; q < 4; q++) a[q] = A.wbf[P_W3 + (4 * g + q) * NH + 16 * mt + r16];
@@ -243,14 +221,14 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs A) {
     acc = mfma16(a, ld4(sDY + my * NOUTP + 4 * g), acc);
     for (int r = 0; r < 4; r++) {
       const int u = 16 * mt + 4 * g + r;
-      const float d = acc[r] * gelu_deriv(h2[mt][r]);
+      const float d = acc[r] * relu_deriv(h2[mt][r]);
       sD2[my * NH + u] = bf16_bits(d);
       const float v = sum16(d);
       if (r16 == 0) sBias[w][NH + u] = v;
     }
   }
   __syncthreads();
-  // dA1 = W2^T dH2, dH1 = dA1 * gelu'(h1)
+  // dA1 = W2^T dH2, dH1 = dA1 * relu'(h1)
   for (int mt = 0; mt < 4; mt++) {
     fx4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int kb = 0; kb < NH / 16; kb++) {
@@ -260,7 +238,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(TrainArgs A) {
     }
     for (int r = 0; r < 4; r++) {
       const int u = 16 * mt + 4 * g + r;
-      const float d = acc[r] * gelu_deriv(h1[mt][r]);
+      const float d = acc[r] * relu_deriv(h1[mt][r]);
       sD1[my * NH + u] = bf16_bits(d);
       const float v = sum16(d);
       if (r16 == 0) sBias[w][u] = v;
@@ -576,7 +554,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PU_SGS_INFE
         acc = mfma32(wa, xb[kb], acc);
       }
 #pragma unroll
-      for (int r = 0; r < 4; r++) a1[mt][r] = bf16_bits(gelu(acc[r]));
+      for (int r = 0; r < 4; r++) a1[mt][r] = bf16_bits(relu(acc[r]));
     }
     // layer 2: the B fragment of K block kb2 is (a1[2 kb2], a1[2 kb2 + 1]) in unit_of_k order
 #pragma unroll
@@ -586,7 +564,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PU_SGS_INFE
       for (int kb = 0; kb < 2; kb++)
         acc = mfma32(ld8(sW2 + (16 * mt + r16) * NH + 32 * kb + 8 * g), cat8(a1[2 * kb], a1[2 * kb + 1]), acc);
 #pragma unroll
-      for (int r = 0; r < 4; r++) a2[mt][r] = bf16_bits(gelu(acc[r]));
+      for (int r = 0; r < 4; r++) a2[mt][r] = bf16_bits(relu(acc[r]));
     }
     fx4 y = *reinterpret_cast<const fx4 *>(sB + 2 * NH + 4 * g);
 #pragma unroll

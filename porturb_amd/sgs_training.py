@@ -137,7 +137,7 @@ def max_abs_input(snap, cells, fmul, max_cells=2000000, seed=0):
 
 
 def init_params(seed=0):
-    """He-scaled normal weights for the GELU layers, Xavier for the output layer, zero biases."""
+    """He-scaled normal weights for the ReLU layers, Xavier for the output layer, zero biases."""
     rng = np.random.default_rng(seed)
     H = pa.SGS_NH
     parts = [rng.standard_normal((H, NIN)) * math.sqrt(2. / NIN), np.zeros(H),

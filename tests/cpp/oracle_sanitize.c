@@ -182,7 +182,7 @@ static void physics_cases(void) {
   CHECK(isfinite(oc_mlp_loss_grad(P, X, T, ns, 1, G)));
   for (int i = 0; i < OC_MLP_NPARAM; i++) G32[i] = (float)G[i];
   oc_mlp_adam(P, G32, m, v, OC_MLP_NPARAM, 1, 1e-3f, 0.9f, 0.999f, 1e-8f);
-  CHECK(isfinite(oc_gelu(0.3)) && isfinite(oc_gelu_deriv(-0.3)) && oc_bf16_round(1.0f) == 1.0f);
+  CHECK(oc_relu(0.3) == 0.3 && oc_relu_deriv(-0.3) == 0. && oc_bf16_round(1.0f) == 1.0f);
   int r = 2, cx = nx / r, cy = ny / r, cz = nz / r;
   size_t nb = (size_t)(cx + 2) * (cy + 2) * (cz + 2);
   double *les = malloc(6 * nb * sizeof(double)), *imm_c = malloc(nb * sizeof(double));
