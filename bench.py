@@ -16,8 +16,9 @@ The dominant sweep's roofline counts the FLOPs the sweep executes on that state:
 reconstruction's branch (the full limiter, the flat-stencil branch that takes the linear
 reconstruction where the stencil's total variation is <= 1e-20, or the linear reconstruction of a
 cell without WENO) at the start and the end of the timed steps; the full-limiter figure is kept
-beside it.  With the default workload (abl_fp64) the same spun-up state is then advanced by the
-FP32 dycore with the TKE closure and with the learned closure: their ms/step, speed-up over FP64
+beside it, and the chip's own ceilings measured in the same run (pu_measure_fma_rate,
+pu_measure_copy_rate: roofline.measured_ceiling).  With the default workload (abl_fp64) the same
+spun-up state is then advanced by the FP32 dycore with the TKE closure and with the learned closure: their ms/step, speed-up over FP64
 and per-field error against the FP64 state after the same steps are in "modes".
 
 abl_fp32_learned: the FP32 dycore with the learned subgrid closure (network file
@@ -277,6 +278,15 @@ def main():
                                ("windmills.apply + " if turbines else "") + "les_closure.apply") if les else "dycore.time_step"},
            "roofline": roof}
     d.destroy()
+    # the same chip's own ceilings beside the published peak: independent FMA chains (FP64 and
+    # non-packed FP32, the forms the kernels are compiled to) and the plain streaming copy
+    ceil = pa.measure_ceilings()
+    fma = ceil["fma_fp64_TFLOPs"] if fp64 else ceil["fma_fp32_TFLOPs"]
+    roof["measured_ceiling"] = dict(ceil, frac_of_fma=round(achieved / fma, 4),
+                                    step_frac_of_copy=round(value * step_bytes / (ceil["copy_GBps"] * 1e9), 4))
+    print("bench: measured FMA %.1f (FP64) / %.1f (FP32) TFLOP/s, copy %.0f GB/s; %s at %.3f of the measured FMA rate"
+          % (ceil["fma_fp64_TFLOPs"], ceil["fma_fp32_TFLOPs"], ceil["copy_GBps"], dom, achieved / fma),
+          file=sys.stderr, flush=True)
 
     if modes:
         # the same spun-up state advanced by the FP32 dycore (compensated rho, rho*theta) with the

@@ -407,6 +407,15 @@ int  pu_les_closure_set_model(pu_ctx *ctx, const pu_mlp *m);
  * [nz][ny][nx] (bit 0 momentum, bit 1 heat from the network), device outputs. */
 int  pu_les_closure_sgs(pu_ctx *ctx, const pu_coupler_state *cs, double dt_phys, float *tau, unsigned char *flags);
 
+/* Ceilings for the rooflines of this library's kernels, measured on the current device (no
+ * counterpart in the reference).  pu_measure_fma_rate: the vector FMA rate (TFLOP/s, an FMA counted
+ * as 2) of independent FMA chains in FP64 or non-packed FP32 (PU_FP64 / PU_FP32).
+ * pu_measure_copy_rate: the HBM rate (GB/s, bytes read + written) of the plain streaming copy that
+ * pu_memcpy_d2d runs, on two device buffers of `bytes` (a multiple of 16, >= 1 MiB) that it
+ * allocates and frees.  Both synchronise the device. */
+int  pu_measure_fma_rate(int precision, double *tflops);
+int  pu_measure_copy_rate(size_t bytes, double *gbps);
+
 /* Library build information (gfx target, precision variants compiled). */
 const char *pu_build_info(void);
 

@@ -92,6 +92,8 @@ def lib():
         "pu_les_closure_set_immersed_halos": (I, [P, P]),
         "pu_fill_device": (I, [P, D, S]),
         "pu_memcpy_d2d": (I, [P, P, S]),
+        "pu_measure_fma_rate": (I, [I, C.POINTER(D)]),
+        "pu_measure_copy_rate": (I, [S, C.POINTER(D)]),
         "pu_driver_create": (I, [I, I, I, I, D, I, I, C.c_ulonglong, I, C.c_char_p, C.POINTER(P)]),
         "pu_driver_step": (I, [P, I]),
         "pu_driver_set_forcing": (I, [P, P]),
@@ -435,6 +437,16 @@ class Dycore:
             self.destroy()
         except Exception:
             pass
+
+
+def measure_ceilings(copy_bytes=2 << 30):
+    """The device's vector FMA rates (TFLOP/s, FP64 and non-packed FP32) and the HBM rate of the
+    plain streaming copy (GB/s read + written), pu_measure_fma_rate / pu_measure_copy_rate."""
+    f64, f32, bw = C.c_double(), C.c_double(), C.c_double()
+    check(lib().pu_measure_fma_rate(PU_FP64, C.byref(f64)))
+    check(lib().pu_measure_fma_rate(PU_FP32, C.byref(f32)))
+    check(lib().pu_measure_copy_rate(copy_bytes, C.byref(bw)))
+    return {"fma_fp64_TFLOPs": f64.value, "fma_fp32_TFLOPs": f32.value, "copy_GBps": bw.value}
 
 
 def case_init(cfg, case_id, seed, cs):

@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libporturb.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["capi.hip", "dycore_kernels.hip", "sweep_zwall_f32.hip", "sweep_pline_f32.hip", "sweep_seg_x_f32.hip", "init_kernels.hip", "les_kernels.hip", "sgs_kernels.hip", "surface_kernels.hip", "column_kernels.hip", "turbine_kernels.hip", "ncio.hip", "driver.cpp"]
+SOURCES = ["capi.hip", "dycore_kernels.hip", "sweep_zwall_f32.hip", "sweep_pline_f32.hip", "sweep_seg_x_f32.hip", "init_kernels.hip", "les_kernels.hip", "sgs_kernels.hip", "surface_kernels.hip", "column_kernels.hip", "turbine_kernels.hip", "ncio.hip", "ceiling_kernels.hip", "driver.cpp"]
 
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

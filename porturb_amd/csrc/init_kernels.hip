@@ -216,17 +216,23 @@ __global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src, uin
     dst[id] = src[id];
 }
 
+int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  long long n16 = (long long)(bytes / 16);
+  long long blocks = (n16 + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, n16);
+  PU_HIP_CHECK(hipGetLastError());
+  return PU_OK;
+}
+
 int copy_device(void *dst, const void *src, size_t bytes) {
   if (!bytes) return PU_OK;
   if ((bytes % 16) || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16)) {
     PU_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return PU_OK;
   }
-  long long n16 = (long long)(bytes / 16);
-  long long blocks = (n16 + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, nullptr, (const uint4 *)src, (uint4 *)dst, n16);
-  PU_HIP_CHECK(hipGetLastError());
+  int rc = launch_copy(dst, src, bytes, nullptr);
+  if (rc) return rc;
   PU_HIP_CHECK(hipDeviceSynchronize());
   return PU_OK;
 }

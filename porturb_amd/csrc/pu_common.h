@@ -183,6 +183,8 @@ int sgs_infer_device(pu_ctx *ctx, double dt);
 int les_init_device(pu_ctx *ctx, const pu_coupler_state *cs);
 int fill_device(double *ptr, double value, size_t count);
 int copy_device(void *dst, const void *src, size_t bytes);
+// k_copy of bytes (a multiple of 16, 16-byte aligned pointers) on stream s, not waited for
+int launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 int les_set_immersed_halos_device(pu_ctx *ctx, const double *halos);
 int diagnostics_device(pu_ctx *ctx, const pu_coupler_state *cs, double *mass, double *maxw, double *maxabsw);
 int case_init_device(const pu_config *cfg, int case_id, unsigned long long seed, pu_coupler_state *cs,

@@ -59,3 +59,15 @@ def test_struct_layout_matches_header():
         a, b = subprocess.run([os.path.join(d, "t")], capture_output=True, text=True).stdout.split()
     assert int(a) == C.sizeof(porturb_amd.PuConfig)
     assert int(b) == C.sizeof(porturb_amd.PuCouplerState)
+
+
+def test_ceiling_measurements_check_their_arguments_without_device():
+    """pu_measure_fma_rate / pu_measure_copy_rate reject bad arguments before any device call."""
+    import ctypes as C
+    L = porturb_amd.lib()
+    out = C.c_double()
+    assert L.pu_measure_fma_rate(porturb_amd.PU_FP64, None) == porturb_amd.PU_ERR_ARG
+    assert L.pu_measure_fma_rate(7, C.byref(out)) == porturb_amd.PU_ERR_ARG
+    assert L.pu_measure_copy_rate(1 << 10, C.byref(out)) == porturb_amd.PU_ERR_ARG
+    assert L.pu_measure_copy_rate((1 << 20) + 8, C.byref(out)) == porturb_amd.PU_ERR_ARG
+    assert "multiple of 16" in L.pu_last_error().decode()
