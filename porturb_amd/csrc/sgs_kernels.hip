@@ -449,15 +449,15 @@ template <class T> struct InferArgs {
   float *tau;
   unsigned char *flags;
 };
+// One group = 128 consecutive cells of an x row (i0..i0+127 at j, k); a block loops over groups.
+// The group's 3x3 neighbourhood rows of (u, v, w, theta), 130 x 3 x 3 points, are staged in LDS
+// with row-contiguous loads; each wave then takes 32 cells as two 16-cell tiles that share every
+// weight fragment it reads from LDS: inputs from LDS, the network on the matrix cores, the outputs
+// stored from the lanes that hold them in the last layer's D fragments, and (lanes 0-31, one cell
+// each) the fallback rule.
+constexpr int GX = 128, GBX = GX + 2, GBN = GBX * 3 * 3;
 
-// One group = 64 consecutive cells of an x row (i0..i0+63 at j, k); a block loops over groups.
-// The group's 3x3 neighbourhood rows of (u, v, w, theta), 66 x 3 x 3 points, are staged in LDS
-// with row-contiguous loads; each wave then takes 16 cells: inputs from LDS, the network on the
-// matrix cores, the outputs stored from the lanes that hold them in the last layer's D fragment,
-// and (lanes 0-15) the fallback rule.
-constexpr int GX = 64, GBX = GX + 2, GBN = GBX * 3 * 3;
-
-// Waves per SIMD the inference kernel is built for (4: 128 VGPRs; 3: up to 168, no spills).
+// Waves per SIMD the inference kernel is built for (3: up to 168 VGPRs, no spills).
 #ifndef PU_SGS_INFER_WAVES
 #define PU_SGS_INFER_WAVES 3
 #endif
@@ -493,104 +493,147 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PU_SGS_INFE
     const unsigned rest = grp / ngx;
     const int i0 = (int)(grp - rest * ngx) * GX, j = (int)(rest % (unsigned)A.ny), k = (int)(rest / (unsigned)A.ny);
     const int lxmax = (int)(A.nxh - i0);        // box x positions inside the array
-    // Staging: the box is 36 rows (quantity q, z offset, y offset) of 66 points; wave w loads rows
-    // w, w+4, ... with one lane per x point (lanes 0, 1 also take points 64, 65), so each row's
-    // address is wave-uniform.  All loads are issued before any LDS write.
+    // Staging: the box is 36 rows (quantity q, z offset, y offset) of 130 points; wave w loads rows
+    // w, w+4, ... with one lane per x point (points lane, 64 + lane; lanes 0, 1 also 128, 129), so
+    // each row's address is wave-uniform.  All loads are issued before any LDS write.
     constexpr int RPW = 36 / 4;
-    T buf[RPW], buf2[RPW];
+    T buf[RPW], buf2[RPW], buf3[RPW];
 #pragma unroll
     for (int rr = 0; rr < RPW; rr++) {
       const int r = w + 4 * rr;
       const int q = r / 9, lz = (r / 3) % 3, ly = r % 3;
       const T *row = A.F + (long long)(1 + q) * A.nh + ((long long)(k + lz) * A.nyh + (j + ly)) * A.nxh + i0;
       buf[rr] = lane < lxmax ? row[lane] : T(0);
-      buf2[rr] = (lane < 2 && 64 + lane < lxmax) ? row[64 + lane] : T(0);
+      buf2[rr] = 64 + lane < lxmax ? row[64 + lane] : T(0);
+      buf3[rr] = (lane < 2 && 128 + lane < lxmax) ? row[128 + lane] : T(0);
     }
     __syncthreads();                            // the previous group is done with sQ
 #pragma unroll
     for (int rr = 0; rr < RPW; rr++) {
       const int r = w + 4 * rr;
       sQ[r * GBX + lane] = buf[rr];
-      if (lane < 2) sQ[r * GBX + 64 + lane] = buf2[rr];
+      sQ[r * GBX + 64 + lane] = buf2[rr];
+      if (lane < 2) sQ[r * GBX + 128 + lane] = buf3[rr];
     }
     __syncthreads();
-    const int xl = 16 * w + r16;                // this lane's cell within the group
-    const int i = i0 + xl;
-    const bool in = i < A.nx;
-    const int c = (1 * 3 + 1) * GBX + xl + 1;  // the cell's index in the staged box
-    // inputs: K block kb, lane group g -> neighbours 8 kb + 2 g and 8 kb + 2 g + 1
-    bfx8 xb[4];
-    float mx = 0.f;
+    // inputs of the two tiles: tile t holds cells 32 w + 16 t + r16; K block kb, lane group g ->
+    // neighbours 8 kb + 2 g and 8 kb + 2 g + 1
+    bfx8 xb[2][4];
+    float mxt[2] = {0.f, 0.f};
 #pragma unroll
-    for (int kb = 0; kb < 4; kb++) {
-      bfx8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int t = 0; t < 2; t++) {
+      const int c = (1 * 3 + 1) * GBX + 32 * w + 16 * t + r16 + 1;   // the cell's index in the staged box
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int n = 8 * kb + 2 * g + h;
-        if (n < 27) {
-          const int dz = n / 9 - 1, dy = (n / 3) % 3 - 1, dx = n % 3 - 1;
-          const int nb = c + (dz * 3 + dy) * GBX + dx;
+      for (int kb = 0; kb < 4; kb++) {
+        bfx8 v = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const float x = (float)(sQ[q * GBN + nb] - sQ[q * GBN + c]) * sFm[4 * n + q];
-            mx = fmaxf(mx, fabsf(x));
-            v[4 * h + q] = bf16_bits(x);
+        for (int h = 0; h < 2; h++) {
+          const int n = 8 * kb + 2 * g + h;
+          if (n < 27) {
+            const int dz = n / 9 - 1, dy = (n / 3) % 3 - 1, dx = n % 3 - 1;
+            const int nb = c + (dz * 3 + dy) * GBX + dx;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              const float x = (float)(sQ[q * GBN + nb] - sQ[q * GBN + c]) * sFm[4 * n + q];
+              mxt[t] = fmaxf(mxt[t], fabsf(x));
+              v[4 * h + q] = bf16_bits(x);
+            }
           }
         }
+        xb[t][kb] = v;
       }
-      xb[kb] = v;
     }
-    mx = fmaxf(mx, __shfl_xor(mx, 16));
-    mx = fmaxf(mx, __shfl_xor(mx, 32));
-    // layer 1: D tiles of units 16 mt .. 16 mt + 15
-    bfx4 a1[4], a2[4];
+    // the largest |input| of each tile's cell r16: the four lane groups hold a quarter of its inputs each
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      mxt[t] = fmaxf(mxt[t], __shfl_xor(mxt[t], 16));
+      mxt[t] = fmaxf(mxt[t], __shfl_xor(mxt[t], 32));
+    }
+    // layer 1: D tiles of units 16 mt .. 16 mt + 15, one weight fragment read for both tiles
+    bfx4 a1[2][4], a2[2][4];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
-      fx4 acc = *reinterpret_cast<const fx4 *>(sB + 16 * mt + 4 * g);   // the bias starts the sum
+      const fx4 bias = *reinterpret_cast<const fx4 *>(sB + 16 * mt + 4 * g);   // the bias starts the sum
+      fx4 acc0 = bias, acc1 = bias;
 #pragma unroll
       for (int kb = 0; kb < 4; kb++) {
         // inputs 112..127 do not exist (their B values are 0): lanes 32..63 of block 3 use a zero A
         const bfx8 wa = (kb == 3 && g >= 2) ? bfx8{0, 0, 0, 0, 0, 0, 0, 0} : ld8(sW1 + (16 * mt + r16) * NINP + 32 * kb + 8 * g);
-        acc = mfma32(wa, xb[kb], acc);
+        acc0 = mfma32(wa, xb[0][kb], acc0);
+        acc1 = mfma32(wa, xb[1][kb], acc1);
       }
 #pragma unroll
-      for (int r = 0; r < 4; r++) a1[mt][r] = bf16_bits(relu(acc[r]));
+      for (int r = 0; r < 4; r++) {
+        a1[0][mt][r] = bf16_bits(relu(acc0[r]));
+        a1[1][mt][r] = bf16_bits(relu(acc1[r]));
+      }
     }
     // layer 2: the B fragment of K block kb2 is (a1[2 kb2], a1[2 kb2 + 1]) in unit_of_k order
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
-      fx4 acc = *reinterpret_cast<const fx4 *>(sB + NH + 16 * mt + 4 * g);
+      const fx4 bias = *reinterpret_cast<const fx4 *>(sB + NH + 16 * mt + 4 * g);
+      fx4 acc0 = bias, acc1 = bias;
 #pragma unroll
-      for (int kb = 0; kb < 2; kb++)
-        acc = mfma32(ld8(sW2 + (16 * mt + r16) * NH + 32 * kb + 8 * g), cat8(a1[2 * kb], a1[2 * kb + 1]), acc);
+      for (int kb = 0; kb < 2; kb++) {
+        const bfx8 wa = ld8(sW2 + (16 * mt + r16) * NH + 32 * kb + 8 * g);
+        acc0 = mfma32(wa, cat8(a1[0][2 * kb], a1[0][2 * kb + 1]), acc0);
+        acc1 = mfma32(wa, cat8(a1[1][2 * kb], a1[1][2 * kb + 1]), acc1);
+      }
 #pragma unroll
-      for (int r = 0; r < 4; r++) a2[mt][r] = bf16_bits(relu(acc[r]));
+      for (int r = 0; r < 4; r++) {
+        a2[0][mt][r] = bf16_bits(relu(acc0[r]));
+        a2[1][mt][r] = bf16_bits(relu(acc1[r]));
+      }
     }
-    fx4 y = *reinterpret_cast<const fx4 *>(sB + 2 * NH + 4 * g);
+    const fx4 b3 = *reinterpret_cast<const fx4 *>(sB + 2 * NH + 4 * g);
+    fx4 y0 = b3, y1 = b3;
 #pragma unroll
-    for (int kb = 0; kb < 2; kb++) y = mfma32(ld8(sW3 + r16 * NH + 32 * kb + 8 * g), cat8(a2[2 * kb], a2[2 * kb + 1]), y);
-    // Lane (r16, g) holds outputs 4 g .. 4 g + 3 of cell r16: denormalise and store them there.
+    for (int kb = 0; kb < 2; kb++) {
+      const bfx8 wa = ld8(sW3 + r16 * NH + 32 * kb + 8 * g);
+      y0 = mfma32(wa, cat8(a2[0][2 * kb], a2[0][2 * kb + 1]), y0);
+      y1 = mfma32(wa, cat8(a2[1][2 * kb], a2[1][2 * kb + 1]), y1);
+    }
+    // Lane (r16, g) holds outputs 4 g .. 4 g + 3 of cell r16 of each tile: denormalise and store them there.
     const fx4 om = *reinterpret_cast<const fx4 *>(sMS + 4 * g), os = *reinterpret_cast<const fx4 *>(sMS + NOUTP + 4 * g);
-    float t4[4];
+    float t40[4], t41[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) t4[r] = __fadd_rn(om[r], __fmul_rn(os[r], y[r]));
-    const long long id = ((long long)k * A.ny + j) * A.nx + i;
-    if (in) {
+    for (int r = 0; r < 4; r++) {
+      t40[r] = __fadd_rn(om[r], __fmul_rn(os[r], y0[r]));
+      t41[r] = __fadd_rn(om[r], __fmul_rn(os[r], y1[r]));
+    }
+    const long long idrow = ((long long)k * A.ny + j) * A.nx;
+    {
+      const int i = i0 + 32 * w + r16;
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        if (4 * g + r < NOUT) taug[r * A.n3 + id] = t4[r];
+        if (4 * g + r < NOUT) {
+          if (i < A.nx) taug[r * A.n3 + idrow + i] = t40[r];
+          if (i + 16 < A.nx) taug[r * A.n3 + idrow + i + 16] = t41[r];
+        }
     }
-    // outputs 4..7 (lane group 1) and 8 (group 2) to lanes 0..15 for the fallback rule
+    // the fallback rule: lane L < 32 takes cell 32 w + L (tile L / 16, cell L % 16), gathering that
+    // cell's 9 outputs from the lanes that hold them
+    const int src = r16;
     float t[NOUT];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      t[r] = t4[r];
-      t[4 + r] = __shfl(t4[r], lane + 16);
+      const float a0 = __shfl(t40[r], src), a1v = __shfl(t41[r], src);
+      const float b0 = __shfl(t40[r], src + 16), b1v = __shfl(t41[r], src + 16);
+      t[r] = g == 0 ? a0 : a1v;
+      t[4 + r] = g == 0 ? b0 : b1v;
     }
-    t[8] = __shfl(t4[0], lane + 32);
-    if (g == 0 && in) {
+    {
+      const float c0 = __shfl(t40[0], src + 32), c1 = __shfl(t41[0], src + 32);
+      t[8] = g == 0 ? c0 : c1;
+    }
+    const float mxl = g == 0 ? mxt[0] : mxt[1];
+    const int xl = 32 * w + lane;                // this lane's cell (lanes 0..31)
+    const int i = i0 + xl;
+    if (lane < 32 && i < A.nx) {
+      const long long id = idrow + i;
+      const int c = (1 * 3 + 1) * GBX + xl + 1;
       unsigned char fl = 0;
-      if (A.valid[id] && mx <= A.xmax) {
+      if (A.valid[id] && mxl <= A.xmax) {
         const T *U = sQ, *V = sQ + GBN, *W = sQ + 2 * GBN, *TH = sQ + 3 * GBN;
         const int sy = GBX, sz = 3 * GBX;
         const double ux = (U[c + 1] - U[c - 1]) * A.r2dx, uy = (U[c + sy] - U[c - sy]) * A.r2dy, uz = (U[c + sz] - U[c - sz]) * A.r2dz;
