@@ -196,8 +196,13 @@ def run_learned(cfg, st, imm, gm, dt):
     return cs.download(), tau, fl
 
 
-def test_inference_matches_oracle():
-    cfg, st, imm = les_case(18)
+@pytest.mark.parametrize("shape,learned_min", [((24, 20, 14), 0.2), ((150, 6, 10), 0.1)], ids=["24x20x14", "150x6x10"])
+def test_inference_matches_oracle(shape, learned_min):
+    """A random network on a closure box with buildings; 150 cells in x span two 128-cell groups of
+    the inference kernel, the second one ending inside a wave's second 16-cell tile (its narrow y
+    and z put more cells next to walls and buildings, hence the lower share of learned cells)."""
+    nx, ny, nz = shape
+    cfg, st, imm = les_case(18, nx, ny, nz)
     P = rand_params(19, scale=0.1)
     om = np.array([0.02, 0, -0.01, 0.02, 0, 0.02, 0, 0, -1e-3], np.float32)
     os_ = np.array([0.01, 0.005, 0.005, 0.01, 0.005, 0.01, 1e-3, 1e-3, 1e-3], np.float32)
@@ -207,7 +212,7 @@ def test_inference_matches_oracle():
     ref = st.copy()
     rtau, rfl = o.les_apply_learned(cfg, ref, imm, dt, om_)
     assert (fl == rfl).mean() >= 0.99
-    assert (rfl == 3).sum() > 0.2 * rfl.size
+    assert (rfl == 3).sum() > learned_min * rfl.size
     for f in range(9):
         s = max(np.abs(rtau[f]).max(), 1e-30)
         err = np.abs(tau[f] - rtau[f])
