@@ -24,8 +24,9 @@ struct LesParams {
   double rdelta, nu_pr, rR_d;      // 1 / delta, nu / Pr (Pr = 0.7), 1 / R_d
   double rdx, rdy, rdz;            // 1 / dx, 1 / dy, 1 / dz
   double R_d, C0, gamma;
-  // learned closure: per-cell stresses/heat fluxes [9][n3] and flags [n3] (null: eddy viscosity only)
-  const float *tau;
+  // learned closure: per-cell stresses/heat fluxes per face direction d, [3][n3] of
+  // (tau_0d, tau_1d, tau_2d, q_d), and flags [n3] (null: eddy viscosity only)
+  const float4 *tau;
   const unsigned char *sflags;
 };
 
@@ -98,7 +99,7 @@ constexpr int LBX = LTX + 2, LBY = LTY + 2, LBZ = LTZ + 2;   // tile + one-cell 
 template <class T> struct TileBox {
   const T *sm;                               // closure fields of the tile + halo, [field][n]
   const uint8_t *icode;                      // immersed codes of the box points: 0 none, 1 partial, 2 full
-  const float *tau;                          // learned closure: global [9][n3] stresses/heat fluxes, or null
+  const float4 *tau;                         // learned closure: global [3][n3] per-direction stresses, or null
   long long n3;
   const unsigned char *flag;                 // learned closure: [n] flags of the box points
   static constexpr int sx = 1, sy = LBX, sz = LBX * LBY, n = LBX * LBY * LBZ;
@@ -198,17 +199,18 @@ __device__ __forceinline__ void face_flux(const TileBox<T> &B, const LesParams &
     // Learned closure (no reference counterpart): on a face whose two cells both hold the
     // network's flag, the momentum fluxes become rho * avg(tau_{i,DIR}) (bit 0) and the theta
     // flux rho * avg(q_DIR) (bit 1).  Box points outside the domain carry flag 0 unless the
-    // boundary is periodic (then they hold the wrapped cell's values).  The stresses are read from
-    // the network's output array directly (neighbouring threads read neighbouring cells).
+    // boundary is periodic (then they hold the wrapped cell's values).  The network's outputs are
+    // stored per face direction, so a face reads one 16-byte vector of each of its two cells.
     const int both = B.flag[m] & B.flag[c];
-    if (both & 1) {
-      for (int f = 0; f < 3; f++) {
-        const int lo_ = f < DIR ? f : DIR, hi_ = f < DIR ? DIR : f;
-        const int ti = lo_ == 0 ? hi_ : (lo_ == 1 ? 2 + hi_ : 5);
-        fc.f[f] = rho * (h * ((T)B.tau[ti * B.n3 + gm] + (T)B.tau[ti * B.n3 + gc]));
+    if (both) {
+      const float4 a = B.tau[DIR * B.n3 + gm], b = B.tau[DIR * B.n3 + gc];
+      if (both & 1) {
+        fc.f[0] = rho * (h * ((T)a.x + (T)b.x));
+        fc.f[1] = rho * (h * ((T)a.y + (T)b.y));
+        fc.f[2] = rho * (h * ((T)a.z + (T)b.z));
       }
+      if (both & 2) fc.f[3] = rho * (h * ((T)a.w + (T)b.w));
     }
-    if (both & 2) fc.f[3] = rho * (h * ((T)B.tau[(6 + DIR) * B.n3 + gm] + (T)B.tau[(6 + DIR) * B.n3 + gc]));
   }
 }
 
@@ -486,7 +488,7 @@ int les_apply_device(pu_ctx *ctx, pu_coupler_state *cs, double dt) {
     LaunchTimer lt(ctx, "sgs");
     rc = sgs_infer_device(ctx, dt);
     if (rc) return rc;
-    P.tau = ctx->d_sgs_tau;
+    P.tau = reinterpret_cast<const float4 *>(ctx->d_sgs_tau);
     P.sflags = ctx->d_sgs_flags;
   }
   {

@@ -76,7 +76,7 @@ struct pu_ctx {
   // learned subgrid closure (attached network, its per-cell outputs and the static mask of cells
   // whose level and 3x3x3 neighbourhood allow it)
   const pu_mlp *sgs_model = nullptr;
-  float   *d_sgs_tau = nullptr;                // [9][n3]
+  float   *d_sgs_tau = nullptr;                // [3][n3][4]: per face direction d, (tau_0d, tau_1d, tau_2d, q_d)
   uint8_t *d_sgs_flags = nullptr;              // [n3]
   uint8_t *d_sgs_valid = nullptr;              // [n3]
   float   *d_sgs_fmul = nullptr;               // [108]

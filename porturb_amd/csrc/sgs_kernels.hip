@@ -446,7 +446,7 @@ template <class T> struct InferArgs {
   float xmax;
   float out_mean[NOUT], out_scale[NOUT];
   double r2dx, r2dy, r2dz, nu_max;
-  float *tau;
+  float4 *tau;          // [3][n3]: per face direction d, (tau_0d, tau_1d, tau_2d, q_d) of each cell
   unsigned char *flags;
 };
 // One group = 128 consecutive cells of an x row (i0..i0+127 at j, k); a block loops over groups.
@@ -481,7 +481,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PU_SGS_INFE
   for (int p = tid; p < NH + NH + NOUTP; p += 256)
     sB[p] = p < NH ? A.master[P_B1 + p] : (p < 2 * NH ? A.master[P_B2 + p - NH] : A.master[P_B3 + p - 2 * NH]);
   for (int p = tid; p < K1; p += 256) sFm[p] = p < NIN ? A.fmul[p] : 0.f;
-  float *const taug = A.tau + (long long)(4 * g) * A.n3;   // this lane group's first output
   if (tid < NOUTP) {
     sMS[tid] = tid < NOUT ? A.out_mean[tid] : 0.f;
     sMS[NOUTP + tid] = tid < NOUT ? A.out_scale[tid] : 0.f;
@@ -602,17 +601,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PU_SGS_INFE
       t41[r] = __fadd_rn(om[r], __fmul_rn(os[r], y1[r]));
     }
     const long long idrow = ((long long)k * A.ny + j) * A.nx;
-    {
-      const int i = i0 + 32 * w + r16;
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (4 * g + r < NOUT) {
-          if (i < A.nx) taug[r * A.n3 + idrow + i] = t40[r];
-          if (i + 16 < A.nx) taug[r * A.n3 + idrow + i + 16] = t41[r];
-        }
-    }
-    // the fallback rule: lane L < 32 takes cell 32 w + L (tile L / 16, cell L % 16), gathering that
-    // cell's 9 outputs from the lanes that hold them
+    // lane L < 32 takes cell 32 w + L (tile L / 16, cell L % 16): it gathers that cell's 9 outputs
+    // from the lanes that hold them, stores them per face direction and applies the fallback rule
     const int src = r16;
     float t[NOUT];
 #pragma unroll
@@ -631,6 +621,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PU_SGS_INFE
     const int i = i0 + xl;
     if (lane < 32 && i < A.nx) {
       const long long id = idrow + i;
+      // outputs t = (tau11, tau12, tau13, tau22, tau23, tau33, q1, q2, q3)
+      A.tau[id] = make_float4(t[0], t[1], t[2], t[6]);
+      A.tau[A.n3 + id] = make_float4(t[1], t[3], t[4], t[7]);
+      A.tau[2 * A.n3 + id] = make_float4(t[2], t[4], t[5], t[8]);
       const int c = (1 * 3 + 1) * GBX + xl + 1;
       unsigned char fl = 0;
       if (A.valid[id] && mxl <= A.xmax) {
@@ -728,6 +722,17 @@ static int mlp_reduce(pu_mlp *m, int nblocks, const float *gin, float *grad_out,
   return PU_OK;
 }
 
+// The network's outputs as tau [9][n3] (tau11, tau12, tau13, tau22, tau23, tau33, q1, q2, q3) from
+// the per-direction layout the closure update reads.
+__global__ __launch_bounds__(256) void k_tau_unpack(const float4 *__restrict__ t4, long long n3, float *__restrict__ tau) {
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n3) return;
+  const float4 a = t4[id], b = t4[n3 + id], c = t4[2 * n3 + id];
+  tau[id] = a.x; tau[n3 + id] = a.y; tau[2 * n3 + id] = a.z;
+  tau[3 * n3 + id] = b.y; tau[4 * n3 + id] = b.z; tau[5 * n3 + id] = c.z;
+  tau[6 * n3 + id] = a.w; tau[7 * n3 + id] = b.w; tau[8 * n3 + id] = c.w;
+}
+
 template <class T>
 static int sgs_infer_t(pu_ctx *ctx, double dt) {
   const pu_config &c = ctx->cfg;
@@ -751,7 +756,7 @@ static int sgs_infer_t(pu_ctx *ctx, double dt) {
   A.r2dx = 1. / (2 * dx); A.r2dy = 1. / (2 * dy); A.r2dz = 1. / (2 * dz);
   const double vmx = 0.25 * dx * dx / dt, vmy = 0.25 * dy * dy / dt, vmz = 0.25 * dz * dz / dt;
   A.nu_max = 0.5 * std::fmin(vmx, std::fmin(vmy, vmz));
-  A.tau = ctx->d_sgs_tau; A.flags = ctx->d_sgs_flags;
+  A.tau = reinterpret_cast<float4 *>(ctx->d_sgs_tau); A.flags = ctx->d_sgs_flags;
   const long long ngroups = (long long)((c.nx + GX - 1) / GX) * c.ny * c.nz;
   if (ngroups >= (1LL << 32)) return fail(PU_ERR_UNSUPPORTED, "learned closure: grid too large for the inference kernel");
   const unsigned grid = (unsigned)std::min<long long>(ngroups, 256LL * 16);
@@ -986,7 +991,7 @@ int pu_les_closure_set_model(pu_ctx *ctx, const pu_mlp *m) {
   if (c.idTKE < 0) return fail(PU_ERR_ARG, "the learned closure needs the LES closure (idTKE)");
   const size_t n3 = (size_t)c.nx * c.ny * c.nz;
   if (!ctx->d_sgs_tau) {
-    PU_HIP_CHECK(hipMalloc(&ctx->d_sgs_tau, sizeof(float) * 9 * n3));
+    PU_HIP_CHECK(hipMalloc(&ctx->d_sgs_tau, sizeof(float) * 12 * n3));
     PU_HIP_CHECK(hipMalloc(&ctx->d_sgs_flags, n3));
     PU_HIP_CHECK(hipMalloc(&ctx->d_sgs_valid, n3));
     PU_HIP_CHECK(hipMalloc(&ctx->d_sgs_fmul, sizeof(float) * NIN));
@@ -1011,7 +1016,11 @@ int pu_les_closure_sgs(pu_ctx *ctx, const pu_coupler_state *cs, double dt_phys, 
   rc = sgs_infer_device(ctx, dt_phys);
   if (rc) return rc;
   const size_t n3 = (size_t)ctx->cfg.nx * ctx->cfg.ny * ctx->cfg.nz;
-  if (tau) PU_HIP_CHECK(hipMemcpyAsync(tau, ctx->d_sgs_tau, sizeof(float) * 9 * n3, hipMemcpyDeviceToDevice, ctx->stream));
+  if (tau) {
+    hipLaunchKernelGGL(k_tau_unpack, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<const float4 *>(ctx->d_sgs_tau), (long long)n3, tau);
+    PU_HIP_CHECK(hipGetLastError());
+  }
   if (flags) PU_HIP_CHECK(hipMemcpyAsync(flags, ctx->d_sgs_flags, n3, hipMemcpyDeviceToDevice, ctx->stream));
   PU_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return PU_OK;
