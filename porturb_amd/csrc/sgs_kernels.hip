@@ -592,7 +592,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PU_SGS_INFE
       y0 = mfma32(wa, cat8(a2[0][2 * kb], a2[0][2 * kb + 1]), y0);
       y1 = mfma32(wa, cat8(a2[1][2 * kb], a2[1][2 * kb + 1]), y1);
     }
-    // Lane (r16, g) holds outputs 4 g .. 4 g + 3 of cell r16 of each tile: denormalise and store them there.
+    // Lane (r16, g) holds outputs 4 g .. 4 g + 3 of cell r16 of each tile: denormalise them there.
     const fx4 om = *reinterpret_cast<const fx4 *>(sMS + 4 * g), os = *reinterpret_cast<const fx4 *>(sMS + NOUTP + 4 * g);
     float t40[4], t41[4];
 #pragma unroll
