@@ -28,21 +28,26 @@ __global__ __launch_bounds__(256) void k_fma_rate(T *out, T a, T b, int iters) {
 template <class T> static int fma_rate(double *tflops) {
   const int blocks = 8192, iters = 4096, reps = 5;
   T *d = nullptr;
-  PU_HIP_CHECK(hipMalloc(&d, sizeof(T) * (size_t)blocks * 256));
-  hipEvent_t e0, e1;
-  PU_HIP_CHECK(hipEventCreate(&e0));
-  PU_HIP_CHECK(hipEventCreate(&e1));
-  hipLaunchKernelGGL(k_fma_rate<T>, dim3(blocks), dim3(256), 0, nullptr, d, (T)0.999, (T)1e-4, iters);   // warm-up
-  PU_HIP_CHECK(hipEventRecord(e0, nullptr));
-  for (int r = 0; r < reps; r++)
-    hipLaunchKernelGGL(k_fma_rate<T>, dim3(blocks), dim3(256), 0, nullptr, d, (T)0.999, (T)1e-4, iters);
-  PU_HIP_CHECK(hipEventRecord(e1, nullptr));
-  PU_HIP_CHECK(hipEventSynchronize(e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   float ms = 0.f;
-  PU_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  PU_HIP_CHECK(hipFree(d));
+  hipError_t e = hipMalloc(&d, sizeof(T) * (size_t)blocks * 256);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_fma_rate<T>, dim3(blocks), dim3(256), 0, nullptr, d, (T)0.999, (T)1e-4, iters);   // warm-up
+    e = hipEventRecord(e0, nullptr);
+  }
+  for (int r = 0; r < reps && e == hipSuccess; r++) {
+    hipLaunchKernelGGL(k_fma_rate<T>, dim3(blocks), dim3(256), 0, nullptr, d, (T)0.999, (T)1e-4, iters);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (d) hipFree(d);
+  if (e != hipSuccess) return fail(PU_ERR_HIP, std::string("pu_measure_fma_rate: ") + hipGetErrorString(e));
   *tflops = 2.0 * FMA_CHAINS * (double)iters * blocks * 256 * reps / (ms * 1e-3) / 1e12;
   return PU_OK;
 }
