@@ -18,7 +18,7 @@ comparison of build/postproc/compare_les.py:11-60):
    subgrid energy k -- and a least-squares rescaled variant, its best case.
 
     python -m porturb_amd.sgs_training --nx 1024 --ny 1024 --nz 256 --dx 1.0 --r 2 \\
-        --snapshots 200 400 --iters 6000 --out porturb_amd/data/sgs_abl_urban_dx2.bin
+        --snapshots 1500 2500 --iters 20000 --batch 8192 --out porturb_amd/data/sgs_abl_urban_dx2.bin
 """
 import argparse
 import json
