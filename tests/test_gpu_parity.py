@@ -10,7 +10,7 @@ import pytest
 
 import porturb_amd as pa
 from oracle import oracle_py
-from tests.helpers import random_state, small_config, rel_norms, isentropic, WV, TKE, P1, P2
+from tests.helpers import random_state, small_config, rel_norms, rounding_sensitivity, isentropic, WV, TKE, P1, P2
 
 pytestmark = pytest.mark.gpu
 
@@ -248,6 +248,53 @@ def test_fp32_mode_tracks_fp64(ord_):
     # density and temperature carry compensated storage; velocities/tracers are plain FP32
     assert nrm[[0, 4]].max() < 1e-6, nrm
     assert nrm.max() < 1e-4, nrm
+
+
+def test_fp32_long_run_tracks_fp64():
+    """1000 steps (dycore + LES closure, WENO-9, buildings, 67 s of flow at up to 3.7 m/s on 50 m
+    cells) of FP32 mode against the FP64 oracle.  The bound at the end is the oracle's own
+    rounding sensitivity on this case: how far its 1000-step result moves, per field, when the
+    initial temperature is perturbed by 6e-8 relative (about one FP32 rounding;
+    tests.helpers.rounding_sensitivity): 2.3e-8 (rho) to 1.9e-5 (w) in relative L2.  FP32 mode must
+    stay within it per field, so rounding to FP32 at every step costs less than one FP32 rounding
+    of the initial state (measured: 0.18-0.64 of it).  After 20 steps rho and rho*theta
+    (compensated storage) agree to 1e-8 and every field to 1e-5 in relative L2."""
+    cfg64 = small_config(nx=32, ny=32, nz=24, ord=9, tracers=[WV, TKE, P1])
+    cfg32 = dict(cfg64, precision=pa.PU_FP32)
+    st, imm = random_state(cfg64, seed=31, buildings=3)
+    print("fp32 long run 32x32x24 ord 9 seed 31", flush=True)
+    d, cs = gpu_setup(cfg32, st, imm)
+    dt = d.compute_time_step()
+
+    def oracle_run(s0, steps, check=None):
+        od = oracle_py.Dycore(cfg64, s0, imm)
+        s = s0.copy()
+        for n in range(steps):
+            od.time_step(s, dt)
+            oracle_py.les_apply(cfg64, s, imm, dt)
+            if check:
+                check(n + 1, s)
+        return s
+
+    def gpu_step_and_check(n, ref):
+        d.time_step(cs, dt)
+        d.les_apply(cs, dt)
+        if n in (20, 200, 1000):
+            nrm = rel_norms(cs.download(), ref)
+            print("step", n, "fp32 vs fp64 per-field [L1, L2, Linf]:\n", nrm, flush=True)
+            assert np.isfinite(nrm).all()
+            if n == 20:
+                assert nrm[[0, 4], 1].max() < 1e-8, nrm
+                assert nrm[:, 1].max() < 1e-5, nrm
+            last[0] = nrm
+
+    last = [None]
+    ref = oracle_run(st, 1000, gpu_step_and_check)
+    d.destroy()
+    sens = rounding_sensitivity(lambda s: oracle_run(s, 1000), st, ref, rel=6e-8)
+    print("oracle sensitivity to 6e-8 of theta, per-field [L1, L2, Linf]:\n", sens, flush=True)
+    print("ratio of L2:", last[0][:, 1] / sens[:, 1], flush=True)
+    assert (last[0][:, 1] < sens[:, 1]).all(), (last[0], sens)
 
 
 def test_run_to_run_bitwise_determinism():
