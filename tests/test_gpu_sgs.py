@@ -219,6 +219,33 @@ def test_inference_matches_oracle(shape, learned_min):
         assert np.median(err) <= 1e-4 * s and err.max() <= 2e-2 * s, f
 
 
+def test_inference_fp32_mode_matches_oracle():
+    """FP32 dycore mode: the closure box is float, so the network's inputs are differences of
+    float-rounded values (the FP64 oracle takes them from doubles).  Differences of nearby theta
+    values of ~300 K lose ~3e-5 K each, which can move an input across a BF16 rounding boundary.
+    The bounds are the FP64 mode's: flags equal on 99 % of the cells, outputs within 1e-4 (median)
+    and 2e-2 (max) of each output's scale (measured: every flag equal, median 0, max 3.7e-3)."""
+    cfg, st, imm = les_case(18)
+    cfg = dict(cfg, precision=pa.PU_FP32)
+    P = rand_params(19, scale=0.1)
+    om = np.array([0.02, 0, -0.01, 0.02, 0, 0.02, 0, 0, -1e-3], np.float32)
+    os_ = np.array([0.01, 0.005, 0.005, 0.01, 0.005, 0.01, 1e-3, 1e-3, 1e-3], np.float32)
+    gm, om_ = model_for(P, om, os_)
+    dt = 0.02
+    _, tau, fl = run_learned(cfg, st, imm, gm, dt)
+    ref = st.copy()
+    rtau, rfl = o.les_apply_learned(dict(cfg, precision=pa.PU_FP64), ref, imm, dt, om_)
+    print("fp32 inference: flags agree on %.4f of the cells" % (fl == rfl).mean(), flush=True)
+    assert (fl == rfl).mean() >= 0.99
+    assert (rfl == 3).sum() > 0.2 * rfl.size
+    both = (fl == rfl)
+    for f in range(9):
+        s = max(np.abs(rtau[f]).max(), 1e-30)
+        err = np.abs(tau[f] - rtau[f])[both]
+        print("output", f, "median %.2e max %.2e of scale" % (np.median(err) / s, err.max() / s), flush=True)
+        assert np.median(err) <= 1e-4 * s and err.max() <= 2e-2 * s, f
+
+
 def neighbourhood_agree(fa, fb):
     """Cells whose own flags and their six face neighbours' flags agree (x/y periodic)."""
     same = fa == fb
