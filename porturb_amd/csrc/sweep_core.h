@@ -785,9 +785,14 @@ template <> struct SegShape<2> { static constexpr int BX = PU_SEG_Z_BX, BY = PU_
 // each cell's final tendency goes straight into the stage combination (combine_field_v) instead of
 // back to `tend`.  The stage runs z (0), y (1), x (2): the last sweep is the one whose rows are
 // 64 cells wide, so the stage-state reads and writes of the combination are coalesced.
+// FP32 x sweep: at most 64 VGPRs (8 waves per SIMD; it fits without spills, and ran 10.75 against
+// 11.21 ms per launch at 72 VGPRs and 7 waves over 205 steps of the 1024x1024x128 urban ABL)
+#ifndef PU_SEG_X_F32_WAVES
+#define PU_SEG_X_F32_WAVES 8
+#endif
 template <int ORD, class T, int DIR, int MODE>
 __global__ __launch_bounds__(SegShape<DIR>::BX * SegShape<DIR>::BY)
-__attribute__((amdgpu_waves_per_eu((std::is_same<T, float>::value && DIR == 0) ? 7 : 1))) void k_sweep(const T *__restrict__ prim, const uint8_t *__restrict__ immf,
+__attribute__((amdgpu_waves_per_eu((std::is_same<T, float>::value && DIR == 0) ? PU_SEG_X_F32_WAVES : 1))) void k_sweep(const T *__restrict__ prim, const uint8_t *__restrict__ immf,
                                                const uint8_t *__restrict__ any10, T *__restrict__ tend,
                                                SweepParams P, FuseArgs<T> FA) {
   constexpr int hs = (ORD - 1) / 2;
